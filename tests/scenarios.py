@@ -1,0 +1,179 @@
+"""Test scenarios shared by the oracle tests and the GPU parity tests.
+
+Each builder restates the setup of one reference test (file:line in the golden
+fixture) or one benchmark configuration, in plain numpy arrays.
+"""
+import json
+import os
+
+import numpy as np
+
+import oracle_path  # noqa: F401  (puts oracle/ on sys.path)
+import oracle as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+
+
+def mesh(dims, bmax, voxel, bmin=(0, 0, 0)):
+    vox = [v if v else 1 for v in voxel]
+    return orc.grid_shape(dims, bmin, bmax, vox), vox
+
+
+def problem_from_env(env, dims, bmax, voxel, quirks=False):
+    grid, vox = mesh(dims, bmax, voxel)
+    return orc.Problem(dims, grid, vox, env["dt"], env["diffusion"], env["decay"],
+                       openmp_quirks=quirks)
+
+
+def constant_field(p, values):
+    f = np.empty(p.shape)
+    f[...] = np.asarray(values, dtype=np.float64)
+    return f
+
+
+def linear_index_field(p):
+    """field[s,x,y,z] = s + x*S + y*S*nx + z*S*nx*ny (test_diffusion_solver.cpp:150-156)."""
+    return np.arange(p.nx * p.ny * p.nz * p.S, dtype=np.float64).reshape(p.shape)
+
+
+# ---------------------------------------------------------------- cell tests
+
+def default_agent(spec, i, offset, volume, position):
+    """set_default_agent_values, test_cell_solver.cpp:104-126."""
+    spec.secretion[i] = [offset + 100, 0]
+    spec.uptake[i] = [offset + 200, 0]
+    spec.saturation[i] = [offset + 300, 0]
+    spec.net_export[i] = [0, offset + 400]
+    spec.fraction_released[i] = [1, 1]
+    spec.volumes[i] = volume
+    spec.positions[i] = position[: spec.dims]
+
+
+def simple_cells(dims):
+    g = GOLDEN["cells"]
+    env = g["default_microenv"]
+    simple = g["Simple"]
+    p = problem_from_env(env, dims, simple["meshes"][str(dims)]["bbox_max"], simple["voxel"])
+    ag = orc.AgentSpec(3, 2, dims)
+    for i, a in enumerate(simple["agents"]):
+        default_agent(ag, i, a["offset"], a["volume"], [a["coord"]] * 3)
+    return p, ag
+
+
+def conflict_cells():
+    g = GOLDEN["cells"]
+    c = g["Conflict"]
+    p = problem_from_env(g["default_microenv"], 1, c["bbox_max"], c["voxel"])
+    ag = orc.AgentSpec(len(c["agents"]), 2, 1)
+    for i, a in enumerate(c["agents"]):
+        default_agent(ag, i, a["offset"], a["volume"], [a["x"], 0, 0])
+    return p, ag
+
+
+def conflict_big_cells():
+    g = GOLDEN["cells"]
+    c = g["ConflictBig"]
+    p = problem_from_env(g["default_microenv"], 1, c["bbox_max"], c["voxel"])
+    k = c["cells_per_voxel"]
+    ag = orc.AgentSpec(p.nx * k, 2, 1)
+    for x in range(p.nx):
+        for j in range(k):
+            # cartesian_mesh::voxel_center (mesh.cpp:63-72)
+            default_agent(ag, x * k + j, c["offset"], c["volume"],
+                          [x * p.voxel[0] + p.voxel[0] / 2.0, 0, 0])
+    return p, ag
+
+
+def brute_force_densities_1d(p, field, ag):
+    """compute_expected_agent_densities_1d, test_cell_solver.cpp:68-102."""
+    V = float(p.voxel[0] * p.voxel[1] * p.voxel[2])
+    out = field.copy().reshape(p.nx, p.S)
+    f = field.reshape(p.nx, p.S)
+    for s in range(p.S):
+        for x in range(p.nx):
+            num = den = fac = 0.0
+            for i in range(ag.n):
+                if int((ag.positions[i, 0] - 0.0) / p.voxel[0]) == x:
+                    num += ag.secretion[i, s] * ag.saturation[i, s] * p.dt * ag.volumes[i] / V
+                    den += (ag.secretion[i, s] + ag.uptake[i, s]) * p.dt * ag.volumes[i] / V
+                    fac += ag.net_export[i, s] * p.dt / V
+            out[x, s] = (f[x, s] + num) / (1 + den) + fac
+    return out
+
+
+def brute_force_internalized_1d(p, field_after, ag, internalized):
+    """compute_expected_agent_internalized_1d, test_cell_solver.cpp:42-66."""
+    V = float(p.voxel[0] * p.voxel[1] * p.voxel[2])
+    f = field_after.reshape(p.nx, p.S)
+    out = internalized.copy()
+    for i in range(ag.n):
+        x = int(ag.positions[i, 0] / p.voxel[0])
+        for s in range(p.S):
+            num = ag.secretion[i, s] * ag.saturation[i, s] * p.dt * ag.volumes[i]
+            den = (ag.secretion[i, s] + ag.uptake[i, s]) * p.dt * ag.volumes[i] / V
+            fac = ag.net_export[i, s] * p.dt
+            out[i, s] -= (V * -den * f[x, s] + num) + fac
+    return out
+
+
+# ------------------------------------------------------- benchmark workloads
+
+def _d_lambda(rng, S):
+    D = 10.0 ** rng.uniform(0.0, 5.0, S)
+    lam = rng.uniform(0.01, 1.0, S)
+    return D, lam
+
+
+def config2(side=200, S=4, n_cells=100_000, seed=43, with_cells=True, with_bulk=True):
+    """Single large domain: side^3 voxels at dx=20, S substrates with D log-uniform in
+    [1,1e5] and decay uniform in [0.01,1], field U[0,1], Dirichlet 1.0 on x-min, bulk
+    supply 0.5 (target 1, uptake 0) inside a central sphere of r=500um, uniform cells
+    with secretion/uptake U[0,10] (saturation 1, volume 2494)."""
+    rng = np.random.default_rng(seed)
+    dx = 20
+    box = side * dx
+    D, lam = _d_lambda(rng, S)
+    p = orc.Problem(3, (side, side, side), (dx, dx, dx), 0.01, D, lam)
+    field = rng.uniform(0.0, 1.0, p.shape)
+    dirs = orc.DirichletSpec(S, 3)
+    dirs.set_face(0, "min", np.ones(S), np.ones(S, dtype=np.uint8))
+    bulk = None
+    if with_bulk:
+        c = (np.arange(side) + 0.5) * dx - box / 2.0
+        zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+        inside = (xx ** 2 + yy ** 2 + zz ** 2) <= 500.0 ** 2
+        supply = np.zeros(p.shape)
+        supply[inside] = 0.5
+        target = np.zeros(p.shape)
+        target[inside] = 1.0
+        bulk = (supply, np.zeros(p.shape), target)
+    ag = None
+    if with_cells and n_cells > 0:
+        ag = orc.AgentSpec(n_cells, S, 3)
+        ag.positions[:] = rng.uniform(0.0, box, (n_cells, 3))
+        np.minimum(ag.positions, np.nextafter(box, 0), out=ag.positions)
+        ag.secretion[:] = rng.uniform(0.0, 10.0, (n_cells, S))
+        ag.uptake[:] = rng.uniform(0.0, 10.0, (n_cells, S))
+        ag.saturation[:] = 1.0
+        ag.volumes[:] = 2494.0
+    return p, field, dirs, bulk, ag
+
+
+def config1(seed=42):
+    """Plain CPU-runnable case: 50^3 at dx=20, S=1, D=1000, decay 0.1, dt=0.01, field 0,
+    Dirichlet 38 on all six faces, 1000 cells uniform (volume 2494, uptake 10,
+    saturation 38)."""
+    rng = np.random.default_rng(seed)
+    p = orc.Problem(3, (50, 50, 50), (20, 20, 20), 0.01, [1000.0], [0.1])
+    field = np.zeros(p.shape)
+    dirs = orc.DirichletSpec(1, 3)
+    for a in range(3):
+        dirs.set_face(a, "min", [38.0], [1])
+        dirs.set_face(a, "max", [38.0], [1])
+    ag = orc.AgentSpec(1000, 1, 3)
+    ag.positions[:] = rng.uniform(0.0, 1000.0, (1000, 3))
+    ag.uptake[:] = 10.0
+    ag.saturation[:] = 38.0
+    ag.volumes[:] = 2494.0
+    return p, field, dirs, None, ag
