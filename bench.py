@@ -1,0 +1,187 @@
+"""Benchmark of the MI355X BioFVM backend.
+
+Default workload (one GPU): the single large domain -- 200^3 voxels at dx=20um,
+4 substrates with D log-uniform in [1,1e5] um^2/min and decay uniform in
+[0.01,1]/min, dt=0.01 min, FP32 field uniform in [0,1], Dirichlet 1.0 on the
+x-min face, bulk supply 0.5/min inside a central sphere of r=500um, 1e5 uniform
+cells with secretion/uptake uniform in [0,10]/min -- stepped 1000 times.  A step
+is one pass of the whole BioFVM step (three Thomas sweeps, Dirichlet, bulk,
+cells) over the resident field; inputs are in HBM before the timed region.
+
+Prints one JSON line: voxel*substrate updates/s, the roofline of the dominant
+kernel (HIP events on the solver stream), and the CPU oracle timed on this
+host's cores as a reported baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def build_c2(side, S, n_cells, seed):
+    """Config-2 inputs, generated without the oracle (same recipe as tests/scenarios.py)."""
+    rng = np.random.default_rng(seed)
+    dx = 20
+    box = side * dx
+    D = 10.0 ** rng.uniform(0.0, 5.0, S)
+    lam = rng.uniform(0.01, 1.0, S)
+    field = rng.uniform(0.0, 1.0, (side, side, side, S))
+    c = (np.arange(side) + 0.5) * dx - box / 2.0
+    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+    inside = ((xx ** 2 + yy ** 2 + zz ** 2) <= 500.0 ** 2).reshape(-1)
+    bulk_vox = np.nonzero(inside)[0]
+    nb = len(bulk_vox)
+    bulk = (bulk_vox, np.full((nb, S), 0.5), np.zeros((nb, S)), np.ones((nb, S)))
+    pos = rng.uniform(0.0, box, (n_cells, 3))
+    np.minimum(pos, np.nextafter(box, 0), out=pos)
+    sec = rng.uniform(0.0, 10.0, (n_cells, S))
+    upt = rng.uniform(0.0, 10.0, (n_cells, S))
+    agents = dict(positions=pos, secretion=sec, saturation=np.ones((n_cells, S)), uptake=upt,
+                  net_export=np.zeros((n_cells, S)), internalized=None,
+                  fraction_released=np.zeros((n_cells, S)), volumes=np.full(n_cells, 2494.0),
+                  is_active=np.ones(n_cells, dtype=np.uint8))
+    return dict(side=side, S=S, D=D, lam=lam, field=field, bulk=bulk, agents=agents, dt=0.01, dx=dx)
+
+
+def gpu_domain(w, precision="fp64"):
+    from physicore_amd import bfvm
+
+    side, S = w["side"], w["S"]
+    d = bfvm.Domain(3, (side, side, side), (w["dx"],) * 3, w["dt"], w["D"], w["lam"], np.zeros(S),
+                    precision=precision)
+    d.upload_field(w["field"])
+    vals = np.zeros((6, S))
+    conds = np.zeros((6, S), dtype=np.uint8)
+    vals[0], conds[0] = 1.0, 1
+    d.set_dirichlet_faces(vals, conds)
+    d.set_bulk(*w["bulk"])
+    a = w["agents"]
+    d.set_agents(a["positions"], a["secretion"], a["saturation"], a["uptake"], a["net_export"],
+                 a["internalized"], a["fraction_released"], a["volumes"], a["is_active"])
+    return d
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """The CPU oracle (FP64 restatement of the OpenMP solver) on this host's cores,
+    on a bounded sample of the same workload: as many whole steps as fit ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    side, S = w["side"], w["S"]
+    p = orc.Problem(3, (side, side, side), (w["dx"],) * 3, w["dt"], w["D"], w["lam"])
+    field = w["field"].copy()
+    dirs = orc.DirichletSpec(S, 3)
+    dirs.set_face(0, "min", np.ones(S), np.ones(S, dtype=np.uint8))
+    vox, sup, upt, tgt = w["bulk"]
+    dense = [np.zeros(p.shape) for _ in range(3)]
+    for arr, src in zip(dense, (sup, upt, tgt)):
+        arr.reshape(-1, S)[vox] = src
+    a = w["agents"]
+    ag = orc.AgentSpec(len(a["volumes"]), S, 3)
+    for k in ("positions", "secretion", "saturation", "uptake", "net_export", "fraction_released",
+              "volumes", "is_active"):
+        setattr(ag, k, np.ascontiguousarray(a[k]))
+    s = orc.Solver(p, field, dirichlet=dirs, bulk=dense, agents=ag)
+    s.solve(1)  # warm-up (includes the recompute), excluded like the reference benchmark
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        s.solve(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s and steps >= 2:
+            break
+    ups = side ** 3 * S * steps / el
+    return {"value": ups, "unit": "voxel*substrate updates/s", "cores": orc.num_threads(),
+            "kind": "port",
+            "sample": f"{steps} whole FP64 steps of the same {side}^3x{S} workload (incl. bulk, "
+                      f"cells, Dirichlet) in {el:.1f}s, after 1 warm-up step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--side", type=int, default=200)
+    ap.add_argument("--substrates", type=int, default=4)
+    ap.add_argument("--cells", type=int, default=100_000)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
+              "(domains shard by HBM budget or z-slabs, described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from physicore_amd import bfvm
+
+    if bfvm.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    w = build_c2(args.side, args.substrates, args.cells, seed=43)
+    d = gpu_domain(w, args.precision)
+    V = args.side ** 3
+    units = V * args.substrates
+
+    d.step_graph(args.warmup, args.graph_steps)
+    d.sync()
+    t0 = time.perf_counter()
+    d.step_graph(args.steps, args.graph_steps)
+    d.sync()
+    el = time.perf_counter() - t0
+    ms_per_step = el * 1e3 / args.steps
+    value = units * args.steps / el
+
+    # per-stage launch durations with HIP events on the solver stream
+    stages = d.time_stages(min(args.steps, 50))
+    sweep_bytes = units * 8  # one sweep reads and writes every FP32 value once
+    dominant = max(("x", "y", "z"), key=lambda k: stages[k])
+    achieved = sweep_bytes / (stages[dominant] * 1e-6) / 1e9
+    diff_us = stages["x"] + stages["y"] + stages["z"]
+    step_gbs = units * 24 / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "voxel*substrate updates/s",
+        "value": value,
+        "unit": "voxel*substrate updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 field, f64 sweep arithmetic" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"single large domain {args.side}^3 voxels x {args.substrates} substrates, "
+                               f"dx=20um, dt=0.01min, x-min Dirichlet, bulk sphere r=500um, "
+                               f"{args.cells} cells, whole BioFVM step",
+                   "graph_steps": args.graph_steps, "seed": 43},
+        "roofline": {"bound": "hbm", "kernel": f"sweep_{dominant}", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "algorithmic_bytes_per_launch": sweep_bytes,
+                     "avg_launch_us": stages[dominant]},
+        "step_roofline": {"bytes_per_step": units * 24, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS,
+                          "diffusion_us": diff_us,
+                          "diffusion_frac": units * 24 / (diff_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
+        "stages_us": stages,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

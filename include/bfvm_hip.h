@@ -1,0 +1,191 @@
+/*
+ * bfvm_hip.h -- C ABI of the MI355X BioFVM backend (libbfvm_hip.so).
+ *
+ * One bfvm_ctx is one BioFVM microenvironment on one GPU: the substrate field,
+ * the Thomas factors, the Dirichlet and bulk hooks and the agent arrays, all
+ * resident in HBM, with one HIP stream.  The C++20 hip_solver
+ * (physicore_amd/host) is the reference-side caller; Python tests and the
+ * benchmark call the same symbols through ctypes.  Plain pointers and sizes
+ * only, no exceptions across the ABI: every call returns BFVM_OK (0) or a
+ * negative bfvm_status, and bfvm_last_error() holds the message.  A context is
+ * not thread-safe, as the reference solvers are not.
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference checkout; BFVM = reactions-diffusion/biofvm):
+ *   biofvm::solver                     BFVM/include/biofvm/solver.h:12-37
+ *   openmp_solver::initialize/solve    BFVM/kernels/openmp_solver/src/openmp_solver.cpp:8-40
+ *   thrust_solver data_manager         BFVM/kernels/thrust_solver/src/data_manager.cpp:11-109
+ */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bfvm_status
+{
+	BFVM_OK = 0,
+	BFVM_ERR_INVALID = -1,	   /* bad argument or descriptor */
+	BFVM_ERR_HIP = -2,		   /* a HIP runtime call failed */
+	BFVM_ERR_NO_MEMORY = -3,   /* device allocation failed or budget exceeded */
+	BFVM_ERR_OUT_OF_GRID = -4, /* an active agent lies outside the mesh */
+	BFVM_ERR_UNSUPPORTED = -5, /* shape beyond what the kernels handle */
+	BFVM_ERR_STATE = -6		   /* call out of order (e.g. no agents set) */
+} bfvm_status;
+
+/* Arithmetic of the Thomas recurrences.  The field is FP32 in HBM in both. */
+typedef enum bfvm_precision
+{
+	BFVM_SWEEP_FP64 = 0, /* FP64 recurrences on FP32 storage (default) */
+	BFVM_SWEEP_FP32 = 1	 /* FP32 recurrences on FP32 factors */
+} bfvm_precision;
+
+/* How the cell stage accumulates the per-voxel sums at recompute. */
+typedef enum bfvm_reduce_mode
+{
+	BFVM_REDUCE_ATOMIC = 0, /* owner election by CAS, wavefront-aggregated FP64 atomics */
+	BFVM_REDUCE_SORTED = 1	/* sort by voxel, segmented reduce: bit-reproducible */
+} bfvm_reduce_mode;
+
+/*
+ * One microenvironment.  Mirrors the fields solver backends read from
+ * biofvm::microenvironment (BFVM/include/biofvm/microenvironment.h:46-81) and
+ * cartesian_mesh (BFVM/include/biofvm/mesh.h:11-32).
+ */
+typedef struct bfvm_domain_desc
+{
+	int32_t dims;		 /* 1, 2 or 3 */
+	int32_t substrates;	 /* S >= 1 */
+	int64_t grid[3];	 /* nx, ny, nz (1 for axes >= dims) */
+	int64_t voxel[3];	 /* dx, dy, dz (integers, as cartesian_mesh::voxel_shape) */
+	int64_t bbox_min[3]; /* mesh bounding box minimum */
+	double dt;			 /* diffusion_timestep */
+	const double* diffusion_coefficients; /* [S] */
+	const double* decay_rates;			  /* [S] */
+	const double* initial_conditions;	  /* [S] */
+	int32_t compute_internalized;		  /* microenvironment::compute_internalized_substrates */
+	int32_t precision;					  /* bfvm_precision */
+	int32_t reduce_mode;				  /* bfvm_reduce_mode */
+	/* 1: reproduce the OpenMP backend's tile-relative y/z factor index and its
+	 * single-voxel-line factor (BFVM/kernels/openmp_solver/src/diffusion_solver.cpp:29-113,390-794);
+	 * 0: per-substrate factors as BFVM/kernels/thrust_solver/src/diffusion_solver.cpp:85-140. */
+	int32_t openmp_quirks;
+} bfvm_domain_desc;
+
+typedef struct bfvm_ctx bfvm_ctx;
+
+/* Message of the last failed call on ctx (or of the last failed bfvm_create
+ * on this thread when ctx is NULL).  Never NULL. */
+const char* bfvm_last_error(const bfvm_ctx* ctx);
+
+/* Number of HIP devices visible (0 when none); never fails. */
+int bfvm_device_count(void);
+
+/* openmp_solver::initialize (openmp_solver.cpp:8-21) + problem_t::construct
+ * (openmp_solver/src/problem.h:27-46): allocates the field, fills it with the
+ * initial conditions, precomputes the Thomas factors in FP64. */
+int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out);
+void bfvm_destroy(bfvm_ctx* ctx);
+
+/* Face Dirichlet conditions, faces ordered x-min, x-max, y-min, y-max, z-min,
+ * z-max; values[6*S], conds[6*S] (0/1).  NULL values or conds turn every face
+ * off.  Replaces dirichlet_solver::initialize / reinitialize_dirichlet
+ * (thrust_solver/src/dirichlet_solver.cpp:192-218; openmp_solver.cpp:58-62)
+ * for the face arrays of microenvironment.h:72-75. */
+int bfvm_set_dirichlet_faces(bfvm_ctx* ctx, const double* values, const uint8_t* conds);
+
+/* Interior Dirichlet voxels (microenvironment.h:65-69): voxels[count*dims]
+ * grid indices, values[count*S], conds[count*S].  Later entries of a repeated
+ * voxel win, as the reference's sequential order does. */
+int bfvm_set_dirichlet_interior(bfvm_ctx* ctx, int64_t count, const int64_t* voxels, const double* values,
+								const uint8_t* conds);
+
+/* Bulk supply/uptake hook (bulk_functor, BFVM/include/biofvm/bulk_functor.h:8-14;
+ * bulk_solver, openmp_solver/src/bulk_solver.cpp:12-49) as a sparse list of
+ * voxels: voxels[count] linear indices x + nx*(y + ny*z), supply/uptake/target
+ * [count*S].  Voxels not listed have S = U = 0, for which the bulk update is
+ * the identity. */
+int bfvm_set_bulk(bfvm_ctx* ctx, int64_t count, const int64_t* voxels, const double* supply, const double* uptake,
+				  const double* target);
+
+/* Agent arrays (agent_data_generic_storage, BFVM/include/biofvm/agent_data_generic_storage.h:16-31),
+ * FP64 as in the reference: positions[n*dims], secretion/saturation/uptake/
+ * net_export/internalized/fraction_released[n*S], volumes[n], is_active[n].
+ * Copies them to HBM (data_manager::transfer_to_device,
+ * thrust_solver/src/data_manager.cpp:61-93) and marks the positional data for
+ * recompute.  internalized may be NULL (zeros). */
+int bfvm_set_agents(bfvm_ctx* ctx, int64_t n, const double* positions, const double* secretion,
+					const double* saturation, const double* uptake, const double* net_export,
+					const double* internalized, const double* fraction_released, const double* volumes,
+					const uint8_t* is_active);
+
+/* solver::recompute_positional_data (openmp_solver.cpp:64): the next step
+ * re-bins the agents and rebuilds the per-voxel sums. */
+int bfvm_recompute_positional_data(bfvm_ctx* ctx);
+
+/* openmp_solver::solve (openmp_solver.cpp:23-40): `iterations` steps of
+ * diffusion -> Dirichlet -> bulk -> cells, enqueued on the context stream.
+ * Returns once enqueued; errors of a recompute are reported synchronously. */
+int bfvm_step(bfvm_ctx* ctx, int64_t iterations);
+
+/* Same, replayed from a HIP graph that captures `graph_steps` steps (the graph
+ * is rebuilt when the positional data or the hooks change). */
+int bfvm_step_graph(bfvm_ctx* ctx, int64_t iterations, int64_t graph_steps);
+
+/* Waits for the context stream. */
+int bfvm_sync(bfvm_ctx* ctx);
+
+/* Field transfers in layout ((z*ny + y)*nx + x)*S + s; FP64 host arrays are
+ * converted (data_manager::transfer_to_host/device, data_manager.cpp:35-93). */
+int bfvm_download_field_f64(bfvm_ctx* ctx, double* host);
+int bfvm_upload_field_f64(bfvm_ctx* ctx, const double* host);
+int bfvm_download_field_f32(bfvm_ctx* ctx, float* host);
+int bfvm_upload_field_f32(bfvm_ctx* ctx, const float* host);
+
+/* Internalized substrates of every agent, [n*S] FP64. */
+int bfvm_download_internalized(bfvm_ctx* ctx, double* host);
+
+/* cell_solver::release_internalized_substrates (openmp_solver/src/cell_solver.cpp:351-374):
+ * D += I*F/V at the agent's voxel, then I = 0.  No-op unless internalized
+ * substrates are tracked. */
+int bfvm_release_internalized(bfvm_ctx* ctx, int64_t agent_index);
+
+/* Total amount per substrate, sum over voxels of density * voxel volume,
+ * reduced in FP64 on the device in a fixed order; out[S]. */
+int bfvm_total_mass(bfvm_ctx* ctx, double* out);
+
+/* Per-owner reduced sums after the last recompute, for parity checks:
+ * owners[n_owners] agent indices (ascending), num/den/fac[n_owners*S] FP64,
+ * *is_conflict.  Pass NULL arrays to query *n_owners only. */
+int bfvm_cell_sums(bfvm_ctx* ctx, int64_t* n_owners, int64_t* owners, double* num, double* den, double* fac,
+				   int32_t* is_conflict);
+
+/* Device pointer of the FP32 field and the context stream (hipStream_t), for
+ * zero-copy callers. */
+int bfvm_field_device_ptr(bfvm_ctx* ctx, void** ptr);
+int bfvm_stream(bfvm_ctx* ctx, void** stream);
+
+/* Kernel timing on the context stream with HIP events: runs `steps` steps and
+ * reports the average duration in microseconds of each stage launch:
+ * out_us[0..5] = x-sweep, y-sweep, z-sweep, interior Dirichlet, bulk, cells
+ * (0 when a stage does not run). */
+int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
+
+/* Launch parameters of the sweep kernels.  persistent_wgs_per_cu = 0 launches
+ * one workgroup per tile; > 0 launches that many workgroups per CU, each
+ * looping over tiles. */
+typedef struct bfvm_tuning
+{
+	int32_t persistent_wgs_per_cu;
+	int32_t yz_columns; /* columns per y/z tile: 64 */
+	int32_t reserved[6];
+} bfvm_tuning;
+int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
+int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,273 @@
+"""ctypes binding of libbfvm_hip.so (include/bfvm_hip.h).
+
+The HIP library is the only compute path: there is no CPU fallback, and every
+call raises BfvmError when the library is missing or a call fails.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "lib", "libbfvm_hip.so")
+HEADER = os.path.join(ROOT, "include", "bfvm_hip.h")
+
+OK = 0
+STATUS = {0: "OK", -1: "INVALID", -2: "HIP", -3: "NO_MEMORY", -4: "OUT_OF_GRID",
+          -5: "UNSUPPORTED", -6: "STATE"}
+PRECISION = {"fp64": 0, "fp32": 1}
+REDUCE = {"atomic": 0, "sorted": 1}
+
+f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
+i64p = C.POINTER(C.c_int64)
+u8p = C.POINTER(C.c_uint8)
+
+
+class BfvmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bfvm {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class DomainDesc(C.Structure):
+    _fields_ = [("dims", C.c_int32), ("substrates", C.c_int32),
+                ("grid", C.c_int64 * 3), ("voxel", C.c_int64 * 3), ("bbox_min", C.c_int64 * 3),
+                ("dt", C.c_double), ("diffusion_coefficients", f64p), ("decay_rates", f64p),
+                ("initial_conditions", f64p), ("compute_internalized", C.c_int32),
+                ("precision", C.c_int32), ("reduce_mode", C.c_int32), ("openmp_quirks", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("yz_columns", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+_LIB = None
+
+_SIGS = {
+    "bfvm_last_error": ([C.c_void_p], C.c_char_p),
+    "bfvm_device_count": ([], C.c_int),
+    "bfvm_create": ([C.POINTER(DomainDesc), C.POINTER(C.c_void_p)], C.c_int),
+    "bfvm_destroy": ([C.c_void_p], None),
+    "bfvm_set_dirichlet_faces": ([C.c_void_p, f64p, u8p], C.c_int),
+    "bfvm_set_dirichlet_interior": ([C.c_void_p, C.c_int64, i64p, f64p, u8p], C.c_int),
+    "bfvm_set_bulk": ([C.c_void_p, C.c_int64, i64p, f64p, f64p, f64p], C.c_int),
+    "bfvm_set_agents": ([C.c_void_p, C.c_int64] + [f64p] * 8 + [u8p], C.c_int),
+    "bfvm_recompute_positional_data": ([C.c_void_p], C.c_int),
+    "bfvm_step": ([C.c_void_p, C.c_int64], C.c_int),
+    "bfvm_step_graph": ([C.c_void_p, C.c_int64, C.c_int64], C.c_int),
+    "bfvm_sync": ([C.c_void_p], C.c_int),
+    "bfvm_download_field_f64": ([C.c_void_p, f64p], C.c_int),
+    "bfvm_upload_field_f64": ([C.c_void_p, f64p], C.c_int),
+    "bfvm_download_field_f32": ([C.c_void_p, f32p], C.c_int),
+    "bfvm_upload_field_f32": ([C.c_void_p, f32p], C.c_int),
+    "bfvm_download_internalized": ([C.c_void_p, f64p], C.c_int),
+    "bfvm_release_internalized": ([C.c_void_p, C.c_int64], C.c_int),
+    "bfvm_total_mass": ([C.c_void_p, f64p], C.c_int),
+    "bfvm_cell_sums": ([C.c_void_p, i64p, i64p, f64p, f64p, f64p, C.POINTER(C.c_int32)], C.c_int),
+    "bfvm_field_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "bfvm_stream": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "bfvm_time_stages": ([C.c_void_p, C.c_int64, f64p], C.c_int),
+    "bfvm_get_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
+    "bfvm_set_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
+}
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in include/bfvm_hip.h."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(bfvm_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise BfvmError(-6, f"{LIB_PATH} is not built; run `python -m physicore_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def device_count():
+    return lib().bfvm_device_count()
+
+
+def _p(a, t=f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+class Domain:
+    """One microenvironment resident on the GPU (a bfvm_ctx)."""
+
+    def __init__(self, dims, grid, voxel, dt, diffusion, decay, initial=None, bbox_min=(0, 0, 0),
+                 compute_internalized=False, precision="fp64", reduce="atomic", openmp_quirks=False):
+        self.dims = int(dims)
+        self.grid = tuple(int(g) for g in grid)
+        self.voxel = tuple(int(v) for v in voxel)
+        self.S = len(diffusion)
+        self._D = _f64(diffusion)
+        self._lam = _f64(decay)
+        self._init = _f64(initial if initial is not None else np.zeros(self.S))
+        desc = DomainDesc()
+        desc.dims = self.dims
+        desc.substrates = self.S
+        for a in range(3):
+            desc.grid[a] = self.grid[a]
+            desc.voxel[a] = self.voxel[a]
+            desc.bbox_min[a] = int(bbox_min[a])
+        desc.dt = float(dt)
+        desc.diffusion_coefficients = _p(self._D)
+        desc.decay_rates = _p(self._lam)
+        desc.initial_conditions = _p(self._init)
+        desc.compute_internalized = int(bool(compute_internalized))
+        desc.precision = PRECISION[precision]
+        desc.reduce_mode = REDUCE[reduce]
+        desc.openmp_quirks = int(bool(openmp_quirks))
+        self._h = C.c_void_p()
+        rc = lib().bfvm_create(C.byref(desc), C.byref(self._h))
+        if rc != OK:
+            raise BfvmError(rc, lib().bfvm_last_error(None).decode())
+        self.n_agents = 0
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc):
+        if rc != OK:
+            raise BfvmError(rc, lib().bfvm_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().bfvm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.grid[2], self.grid[1], self.grid[0], self.S)
+
+    # ---------------------------------------------------------------- hooks
+    def set_dirichlet_faces(self, values, conds):
+        """values/conds: [6, S], faces x-min, x-max, y-min, y-max, z-min, z-max."""
+        v = _f64(values, (6 * self.S,))
+        c = np.ascontiguousarray(conds, dtype=np.uint8).reshape(6 * self.S)
+        self._check(lib().bfvm_set_dirichlet_faces(self._h, _p(v), _p(c, u8p)))
+
+    def set_dirichlet_interior(self, voxels, values, conds):
+        vox = np.ascontiguousarray(voxels, dtype=np.int64).reshape(-1)
+        n = vox.size // self.dims
+        v = _f64(values, (n * self.S,))
+        c = np.ascontiguousarray(conds, dtype=np.uint8).reshape(n * self.S)
+        self._check(lib().bfvm_set_dirichlet_interior(self._h, n, _p(vox, i64p), _p(v), _p(c, u8p)))
+
+    def set_bulk(self, voxels, supply, uptake, target):
+        vox = np.ascontiguousarray(voxels, dtype=np.int64).reshape(-1)
+        n = vox.size
+        arrs = [_f64(a, (n * self.S,)) for a in (supply, uptake, target)]
+        self._check(lib().bfvm_set_bulk(self._h, n, _p(vox, i64p), *(_p(a) for a in arrs)))
+
+    def set_agents(self, positions, secretion, saturation, uptake, net_export, internalized,
+                   fraction_released, volumes, is_active):
+        n = len(volumes)
+        arrs = [_f64(a) for a in (positions, secretion, saturation, uptake, net_export)]
+        internal = None if internalized is None else _f64(internalized)
+        rest = [_f64(fraction_released), _f64(volumes)]
+        act = np.ascontiguousarray(is_active, dtype=np.uint8)
+        self._check(lib().bfvm_set_agents(self._h, n, *(_p(a) for a in arrs), _p(internal),
+                                          *(_p(a) for a in rest), _p(act, u8p)))
+        self.n_agents = n
+
+    def recompute_positional_data(self):
+        self._check(lib().bfvm_recompute_positional_data(self._h))
+
+    # ----------------------------------------------------------------- step
+    def step(self, iterations=1):
+        self._check(lib().bfvm_step(self._h, int(iterations)))
+
+    def step_graph(self, iterations, graph_steps=10):
+        self._check(lib().bfvm_step_graph(self._h, int(iterations), int(graph_steps)))
+
+    def sync(self):
+        self._check(lib().bfvm_sync(self._h))
+
+    # ------------------------------------------------------------ transfers
+    def field(self, dtype=np.float32):
+        if dtype == np.float32:
+            out = np.empty(self.shape, dtype=np.float32)
+            self._check(lib().bfvm_download_field_f32(self._h, _p(out, f32p)))
+        else:
+            out = np.empty(self.shape, dtype=np.float64)
+            self._check(lib().bfvm_download_field_f64(self._h, _p(out)))
+        return out
+
+    def upload_field(self, arr):
+        a = np.ascontiguousarray(arr)
+        if a.size != int(np.prod(self.shape)):
+            raise ValueError("field size mismatch")
+        if a.dtype == np.float32:
+            self._check(lib().bfvm_upload_field_f32(self._h, _p(a, f32p)))
+        else:
+            a = _f64(a)
+            self._check(lib().bfvm_upload_field_f64(self._h, _p(a)))
+
+    def internalized(self):
+        out = np.zeros((self.n_agents, self.S))
+        self._check(lib().bfvm_download_internalized(self._h, _p(out)))
+        return out
+
+    def release_internalized(self, agent):
+        self._check(lib().bfvm_release_internalized(self._h, int(agent)))
+
+    def total_mass(self):
+        out = np.zeros(self.S)
+        self._check(lib().bfvm_total_mass(self._h, _p(out)))
+        return out
+
+    def cell_sums(self):
+        n = C.c_int64(0)
+        conflict = C.c_int32(0)
+        self._check(lib().bfvm_cell_sums(self._h, C.byref(n), None, None, None, None, C.byref(conflict)))
+        k = n.value
+        owners = np.zeros(k, dtype=np.int64)
+        num, den, fac = (np.zeros(k * self.S) for _ in range(3))
+        self._check(lib().bfvm_cell_sums(self._h, C.byref(n), _p(owners, i64p), _p(num), _p(den), _p(fac),
+                                         C.byref(conflict)))
+        r = (k, self.S)
+        return owners, num.reshape(r), den.reshape(r), fac.reshape(r), bool(conflict.value)
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        self._check(lib().bfvm_field_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def time_stages(self, steps):
+        out = np.zeros(6)
+        self._check(lib().bfvm_time_stages(self._h, int(steps), _p(out)))
+        return dict(zip(("x", "y", "z", "dirichlet", "bulk", "cells"), out.tolist()))
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().bfvm_get_tuning(self._h, C.byref(t)))
+        return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu, "yz_columns": t.yz_columns}
+
+    def set_tuning(self, persistent_wgs_per_cu=0, yz_columns=64):
+        t = Tuning()
+        t.persistent_wgs_per_cu = int(persistent_wgs_per_cu)
+        t.yz_columns = int(yz_columns)
+        self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))

@@ -1,0 +1,60 @@
+"""Build recipe of the native parts, run in-tree (no JIT cache):
+
+  physicore_amd/lib/libbfvm_hip.so      HIP kernels + C ABI (hipcc, gfx950)
+  physicore_amd/lib/libbfvm_host.so     C++20 hip_solver + biofvm boundary (g++)
+  oracle/liboracle.so                   CPU oracle (test infrastructure, make)
+
+`python -m physicore_amd.build` or physicore_amd.build.build_all().
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_DIR = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PHYSICORE_AMD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared",
+    # no packed FP32 instructions in any kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall",
+]
+
+
+def run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in r.stdout.splitlines() if "packed-fp32-ops" not in line)
+    if out.strip():
+        print(out, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_hip():
+    os.makedirs(LIB_DIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, f) for f in ("bfvm_kernels.hip", "bfvm_context.hip")]
+    out = os.path.join(LIB_DIR, "libbfvm_hip.so")
+    run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle():
+    run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all():
+    build_hip()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

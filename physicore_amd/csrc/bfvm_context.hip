@@ -1,0 +1,1037 @@
+// bfvm_context.hip -- the C ABI of libbfvm_hip.so (include/bfvm_hip.h): one
+// BioFVM microenvironment resident in HBM, stepped on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bfvm_hip.h"
+#include "bfvm_internal.h"
+
+using namespace bfvm;
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename T>
+struct dvec
+{
+	T* p = nullptr;
+	size_t n = 0;
+	dvec() = default;
+	dvec(const dvec&) = delete;
+	dvec& operator=(const dvec&) = delete;
+	~dvec() { release(); }
+	void release()
+	{
+		if (p)
+			(void)hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+	hipError_t alloc(size_t count)
+	{
+		if (count == n && p)
+			return hipSuccess;
+		release();
+		if (count == 0)
+			return hipSuccess;
+		hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+		if (e != hipSuccess)
+		{
+			p = nullptr;
+			return e;
+		}
+		n = count;
+		return hipSuccess;
+	}
+	hipError_t upload(const T* h, size_t count, hipStream_t s)
+	{
+		hipError_t e = alloc(count);
+		if (e != hipSuccess || count == 0)
+			return e;
+		e = hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+		if (e != hipSuccess)
+			return e;
+		return hipStreamSynchronize(s);
+	}
+};
+
+// Thomas factors of one axis in FP64, openmp_solver/src/diffusion_solver.cpp:29-86
+// (copies = 1); a single-voxel line keeps only the decay share on its diagonal
+// (thrust_solver/src/diffusion_solver.cpp:89-100) unless the OpenMP formula is asked for.
+void precompute_factors(int n, int S, int64_t shape, int dims, double dt, const double* D, const double* lam,
+						bool openmp_quirks, std::vector<double>& b, std::vector<double>& e, std::vector<double>& c)
+{
+	const double h2 = (double)(shape * shape);
+	b.assign((size_t)n * S, 0.0);
+	e.assign((size_t)std::max(n - 1, 1) * S, 0.0);
+	c.assign(S, 0.0);
+	for (int s = 0; s < S; s++)
+		c[s] = dt * D[s] / h2;
+	if (n == 1 && !openmp_quirks)
+	{
+		for (int s = 0; s < S; s++)
+			b[s] = 1 / (1 + lam[s] * dt / (double)dims);
+		return;
+	}
+	for (int i = 0; i < n; i++)
+		for (int s = 0; s < S; s++)
+		{
+			double v = 1 + lam[s] * dt / (double)dims + 2 * dt * D[s] / h2;
+			if (i == 0 || i == n - 1)
+				v -= dt * D[s] / h2;
+			b[(size_t)i * S + s] = v;
+		}
+	for (int s = 0; s < S; s++)
+		b[s] = 1 / b[s];
+	for (int i = 1; i < n; i++)
+		for (int s = 0; s < S; s++)
+		{
+			b[(size_t)i * S + s] = 1 / (b[(size_t)i * S + s] - c[s] * c[s] * b[(size_t)(i - 1) * S + s]);
+			e[(size_t)(i - 1) * S + s] = c[s] * b[(size_t)(i - 1) * S + s];
+		}
+}
+
+// The OpenMP backend's y/z coefficient index (openmp_solver/src/diffusion_solver.cpp:390-401, :427):
+// blocks of S*ceil(48/S) columns, tiles of 48, diagonals indexed tile-relative.
+int32_t openmp_coef_sub(int64_t c, int S, int64_t L)
+{
+	const int64_t copies = (48 + S - 1) / S;
+	const int64_t B = S * copies;
+	const int64_t nb = L / B;
+	const int64_t r = (c < nb * B) ? c % B : c - nb * B;
+	return (int32_t)((r % 48) % S);
+}
+
+bool fp_ok(double v) { return std::isfinite(v); }
+
+} // namespace
+
+struct bfvm_ctx
+{
+	std::string err;
+	int dims = 3, S = 1;
+	int64_t n[3] = { 1, 1, 1 };
+	int64_t voxel[3] = { 1, 1, 1 };
+	int64_t bbox_min[3] = { 0, 0, 0 };
+	int64_t V = 1;
+	int L = 1;
+	double dt = 0.01;
+	bool internalized = false;
+	bool fp64 = true;
+	int reduce_mode = BFVM_REDUCE_ATOMIC;
+	bool quirks = false;
+	std::vector<double> D, lam, init;
+
+	hipStream_t stream = nullptr;
+	int num_cu = 256;
+
+	dvec<float> field;
+	dvec<double> fb[3], fe[3], fc[3];
+	dvec<int32_t> coef_map;
+	dvec<float> face_val;
+	dvec<uint8_t> face_cond;
+	int face_mask = 0;
+
+	int64_t dir_count = 0;
+	dvec<int64_t> dir_vox;
+	dvec<float> dir_val;
+	dvec<uint8_t> dir_cond;
+	dvec<SparseRef> dir_refs;
+
+	int64_t bulk_count = 0;
+	dvec<int64_t> bulk_vox;
+	dvec<double> bulk_a, bulk_b;
+	dvec<SparseRef> bulk_refs;
+
+	int64_t n_agents = 0;
+	dvec<double> positions, secretion, saturation, uptake, net_export, internalized_d, fraction_released, volumes;
+	dvec<uint8_t> is_active;
+	std::vector<double> h_positions;
+	dvec<int32_t> cell_voxel, owner_of, ballot, owners, flags;
+	dvec<double> num, den, fac, rnum, rden, rfac, dnew;
+	dvec<SparseRef> agent_refs; // (0, k) for k < n_agents
+	dvec<uint32_t> keys, vals, tkeys, tvals;
+	dvec<int32_t> hist, scan_tmp, heads, head_scan;
+	bool recompute_pending = false;
+	int64_t n_owners = 0;
+	bool conflict = false;
+
+	dvec<TileRef> tiles[3];
+	int ntiles[3] = { 0, 0, 0 };
+	int fac_cap[3] = { 0, 0, 0 };
+	size_t lds[3] = { 0, 0, 0 };
+	bfvm_tuning tuning {};
+
+	DomainDev hdom {};
+	dvec<DomainDev> ddom;
+
+	dvec<double> mass_partial, mass_out;
+
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t gexec = nullptr;
+	int64_t graph_steps = 0;
+	uint64_t graph_version = ~0ull;
+	uint64_t version = 0;
+
+	int fail(int code, const std::string& m)
+	{
+		err = m;
+		return code;
+	}
+	int hip_fail(hipError_t e, const char* what)
+	{
+		err = std::string(what) + ": " + hipGetErrorString(e);
+		return e == hipErrorOutOfMemory ? BFVM_ERR_NO_MEMORY : BFVM_ERR_HIP;
+	}
+	void drop_graph()
+	{
+		if (gexec)
+			(void)hipGraphExecDestroy(gexec);
+		if (graph)
+			(void)hipGraphDestroy(graph);
+		gexec = nullptr;
+		graph = nullptr;
+		graph_version = ~0ull;
+	}
+	~bfvm_ctx()
+	{
+		if (stream)
+			(void)hipStreamSynchronize(stream);
+		drop_graph();
+		if (stream)
+			(void)hipStreamDestroy(stream);
+	}
+};
+
+#define TRY_HIP(ctx, expr)                               \
+	do                                                   \
+	{                                                    \
+		hipError_t e_ = (expr);                          \
+		if (e_ != hipSuccess)                            \
+			return (ctx)->hip_fail(e_, #expr);           \
+	} while (0)
+
+namespace {
+
+int sync_domain(bfvm_ctx* c)
+{
+	DomainDev& d = c->hdom;
+	d.field = c->field.p;
+	d.nx = (int32_t)c->n[0];
+	d.ny = (int32_t)c->n[1];
+	d.nz = (int32_t)c->n[2];
+	d.S = c->S;
+	d.dims = c->dims;
+	d.L = c->L;
+	d.face_mask = c->face_mask;
+	d.dt = c->dt;
+	d.voxel_volume = (double)(c->voxel[0] * c->voxel[1] * c->voxel[2]);
+	for (int a = 0; a < 3; a++)
+	{
+		d.bbox_min[a] = c->bbox_min[a];
+		d.voxel[a] = c->voxel[a];
+		d.fb[a] = c->fb[a].p;
+		d.fe[a] = c->fe[a].p;
+		d.fc[a] = c->fc[a].p;
+	}
+	d.coef_map = c->coef_map.p;
+	d.face_val = c->face_val.p;
+	d.face_cond = c->face_cond.p;
+	d.dir_vox = c->dir_vox.p;
+	d.dir_val = c->dir_val.p;
+	d.dir_cond = c->dir_cond.p;
+	d.bulk_vox = c->bulk_vox.p;
+	d.bulk_a = c->bulk_a.p;
+	d.bulk_b = c->bulk_b.p;
+	d.n_agents = c->n_agents;
+	d.positions = c->positions.p;
+	d.secretion = c->secretion.p;
+	d.saturation = c->saturation.p;
+	d.uptake = c->uptake.p;
+	d.net_export = c->net_export.p;
+	d.internalized = c->internalized_d.p;
+	d.fraction_released = c->fraction_released.p;
+	d.volumes = c->volumes.p;
+	d.is_active = c->is_active.p;
+	d.cell_voxel = c->cell_voxel.p;
+	d.owner_of = c->owner_of.p;
+	d.ballot = c->ballot.p;
+	d.num = c->num.p;
+	d.den = c->den.p;
+	d.fac = c->fac.p;
+	d.rnum = c->rnum.p;
+	d.rden = c->rden.p;
+	d.rfac = c->rfac.p;
+	d.dnew = c->dnew.p;
+	d.owners = c->owners.p;
+	d.flags = c->flags.p;
+	d.compute_internalized = c->internalized ? 1 : 0;
+	TRY_HIP(c, c->ddom.upload(&d, 1, c->stream));
+	c->version++;
+	return BFVM_OK;
+}
+
+int build_tiles(bfvm_ctx* c)
+{
+	for (int axis = 0; axis < 3; axis++)
+	{
+		std::vector<TileRef> t;
+		if (axis < c->dims)
+		{
+			const int n = (int)c->n[axis];
+			if (axis == 0)
+			{
+				const int R = x_rows_per_tile(c->S);
+				const int rows = (int)(c->n[1] * c->n[2]);
+				for (int r0 = 0; r0 < rows; r0 += R)
+					t.push_back({ 0, r0, std::min(R, rows - r0), 0 });
+			}
+			else
+			{
+				const int outer = (int)(axis == 1 ? c->n[2] : c->n[1]);
+				for (int o = 0; o < outer; o++)
+					for (int c0 = 0; c0 < c->L; c0 += kLinesPerTile)
+						t.push_back({ 0, o, c0, 0 });
+			}
+			c->fac_cap[axis] = n * c->S;
+			c->lds[axis] = sweep_lds_bytes(axis, n, c->S, c->L, c->fac_cap[axis]);
+			if (c->lds[axis] > 160 * 1024)
+				return c->fail(BFVM_ERR_UNSUPPORTED, "grid axis " + std::to_string(axis) + " of length " +
+														 std::to_string(n) + " needs more than 160 KiB of LDS per tile");
+		}
+		c->ntiles[axis] = (int)t.size();
+		TRY_HIP(c, c->tiles[axis].upload(t.data(), t.size(), c->stream));
+	}
+	return BFVM_OK;
+}
+
+int sweep_grid(const bfvm_ctx* c, int axis)
+{
+	if (c->tuning.persistent_wgs_per_cu > 0)
+		return std::min(c->ntiles[axis], c->num_cu * c->tuning.persistent_wgs_per_cu);
+	return c->ntiles[axis];
+}
+
+// One step on the stream: sweeps x, y, z (faces fused into the last), interior
+// Dirichlet, bulk, cells.  Stage order of openmp_solver.cpp:27-37.  `ev` (7
+// events or null) brackets the stages for timing.
+int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
+{
+	auto mark = [&](int k) -> hipError_t { return ev ? hipEventRecord(ev[k], c->stream) : hipSuccess; };
+	TRY_HIP(c, mark(0));
+	for (int axis = 0; axis < 3; axis++)
+	{
+		if (axis < c->dims)
+			TRY_HIP(c, launch_sweep(axis, c->fp64, c->ddom.p, c->tiles[axis].p, c->ntiles[axis],
+									axis == c->dims - 1 ? 1 : 0, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
+									c->stream));
+		TRY_HIP(c, mark(axis + 1));
+	}
+	if (c->dir_count)
+		TRY_HIP(c, launch_dirichlet_interior(c->ddom.p, c->dir_refs.p, c->dir_count, c->stream));
+	TRY_HIP(c, mark(4));
+	if (c->bulk_count)
+		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_refs.p, c->bulk_count, c->stream));
+	TRY_HIP(c, mark(5));
+	if (c->n_owners > 0)
+	{
+		TRY_HIP(c, launch_cells_apply(c->ddom.p, c->agent_refs.p, c->n_owners, c->stream));
+		if (c->internalized && c->conflict)
+			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->agent_refs.p, c->n_agents, c->stream));
+	}
+	TRY_HIP(c, mark(6));
+	return BFVM_OK;
+}
+
+int bits_for(uint64_t v)
+{
+	int b = 0;
+	while (b < 32 && (v >> b) != 0)
+		b++;
+	return std::max(b, 1);
+}
+
+// Re-bins the agents and rebuilds the per-voxel sums (the recompute branch of
+// cell_solver::simulate_secretion_and_uptake, openmp_solver/src/cell_solver.cpp:302-311, :253-267).
+int recompute(bfvm_ctx* c)
+{
+	c->recompute_pending = false;
+	c->n_owners = 0;
+	c->conflict = false;
+	const int64_t n = c->n_agents;
+	if (n == 0)
+		return BFVM_OK;
+	hipStream_t s = c->stream;
+	TRY_HIP(c, hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int32_t), s));
+	if (c->reduce_mode == BFVM_REDUCE_ATOMIC)
+	{
+		if (!c->ballot.p)
+		{
+			TRY_HIP(c, c->ballot.alloc((size_t)c->V));
+			int rc = sync_domain(c);
+			if (rc)
+				return rc;
+		}
+		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, true, s));
+		TRY_HIP(c, launch_cells_ballot(c->ddom.p, n, s));
+		TRY_HIP(c, launch_cells_collect_owners(c->ddom.p, n, s));
+	}
+	else
+	{
+		TRY_HIP(c, c->keys.alloc(n));
+		TRY_HIP(c, c->vals.alloc(n));
+		TRY_HIP(c, c->tkeys.alloc(n));
+		TRY_HIP(c, c->tvals.alloc(n));
+		TRY_HIP(c, c->heads.alloc(n));
+		TRY_HIP(c, c->head_scan.alloc(n));
+		TRY_HIP(c, c->hist.alloc(radix_hist_entries(n)));
+		const size_t st = std::max(scan_tmp_entries(n), scan_tmp_entries((int64_t)radix_hist_entries(n)));
+		TRY_HIP(c, c->scan_tmp.alloc(st));
+		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, false, s));
+		TRY_HIP(c, launch_cells_make_keys(c->ddom.p, c->keys.p, c->vals.p, n, s));
+		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, n,
+									bits_for((uint64_t)c->V), s));
+		TRY_HIP(c, launch_cells_heads(c->keys.p, c->heads.p, n, (uint32_t)c->V, s));
+		TRY_HIP(c, hipMemcpyAsync(c->head_scan.p, c->heads.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+		TRY_HIP(c, exclusive_scan(c->head_scan.p, c->scan_tmp.p, n, s));
+		TRY_HIP(c, launch_cells_segment_reduce(c->ddom.p, c->keys.p, c->vals.p, c->heads.p, c->head_scan.p, n, s));
+	}
+	int32_t f[4] = { 0, 0, 0, 0 };
+	TRY_HIP(c, hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, s));
+	TRY_HIP(c, hipStreamSynchronize(s));
+	if (f[1])
+	{
+		c->recompute_pending = true;
+		return c->fail(BFVM_ERR_OUT_OF_GRID, "an active agent lies outside the mesh");
+	}
+	c->conflict = f[0] != 0;
+	c->n_owners = f[2];
+	return BFVM_OK;
+}
+
+int valid_desc(const bfvm_domain_desc* d, std::string& why)
+{
+	if (!d)
+		return why = "null descriptor", 0;
+	if (d->dims < 1 || d->dims > 3)
+		return why = "dims must be 1, 2 or 3", 0;
+	if (d->substrates < 1 || d->substrates > 64)
+		return why = "substrates must be in [1, 64]", 0;
+	for (int a = 0; a < 3; a++)
+	{
+		if (d->grid[a] < 1)
+			return why = "grid sizes must be >= 1", 0;
+		if (a >= d->dims && d->grid[a] != 1)
+			return why = "grid size of an axis beyond dims must be 1", 0;
+		if (a < d->dims && d->voxel[a] < 1)
+			return why = "voxel sizes must be >= 1", 0;
+	}
+	if ((double)d->grid[0] * d->grid[1] * d->grid[2] > 2147483647.0)
+		return why = "more than 2^31-1 voxels", 0;
+	if ((double)d->grid[0] * d->substrates > 2147483647.0)
+		return why = "row too long", 0;
+	if (!(d->dt > 0) || !fp_ok(d->dt))
+		return why = "dt must be positive", 0;
+	if (!d->diffusion_coefficients || !d->decay_rates || !d->initial_conditions)
+		return why = "null coefficient arrays", 0;
+	for (int s = 0; s < d->substrates; s++)
+		if (!fp_ok(d->diffusion_coefficients[s]) || !fp_ok(d->decay_rates[s]) || !fp_ok(d->initial_conditions[s]) ||
+			d->diffusion_coefficients[s] < 0 || d->decay_rates[s] < 0)
+			return why = "coefficients must be finite and non-negative", 0;
+	if (d->precision != BFVM_SWEEP_FP64 && d->precision != BFVM_SWEEP_FP32)
+		return why = "unknown precision", 0;
+	if (d->reduce_mode != BFVM_REDUCE_ATOMIC && d->reduce_mode != BFVM_REDUCE_SORTED)
+		return why = "unknown reduce mode", 0;
+	return 1;
+}
+
+bool voxel_of(const bfvm_ctx* c, const double* pos, int64_t* v)
+{
+	int64_t idx[3] = { 0, 0, 0 };
+	for (int a = 0; a < c->dims; a++)
+	{
+		const double q = (pos[a] - (double)c->bbox_min[a]) / (double)c->voxel[a];
+		if (!(q >= 0.0) || q >= (double)c->n[a])
+			return false;
+		idx[a] = (int64_t)q;
+	}
+	*v = idx[0] + c->n[0] * (idx[1] + c->n[1] * idx[2]);
+	return true;
+}
+
+std::vector<SparseRef> refs_for(int64_t count)
+{
+	std::vector<SparseRef> r((size_t)count);
+	for (int64_t k = 0; k < count; k++)
+		r[(size_t)k] = { 0, (int32_t)k };
+	return r;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* bfvm_last_error(const bfvm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+int bfvm_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess)
+		return 0;
+	return n;
+}
+
+int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
+{
+	if (!out)
+		return g_error = "null output pointer", BFVM_ERR_INVALID;
+	*out = nullptr;
+	std::string why;
+	if (!valid_desc(desc, why))
+		return g_error = why, BFVM_ERR_INVALID;
+
+	auto* c = new bfvm_ctx;
+	c->dims = desc->dims;
+	c->S = desc->substrates;
+	for (int a = 0; a < 3; a++)
+	{
+		c->n[a] = desc->grid[a];
+		c->voxel[a] = a < desc->dims ? desc->voxel[a] : std::max<int64_t>(desc->voxel[a], 1);
+		c->bbox_min[a] = desc->bbox_min[a];
+	}
+	c->V = c->n[0] * c->n[1] * c->n[2];
+	c->L = (int)(c->n[0] * c->S);
+	c->dt = desc->dt;
+	c->internalized = desc->compute_internalized != 0;
+	c->fp64 = desc->precision == BFVM_SWEEP_FP64;
+	c->reduce_mode = desc->reduce_mode;
+	c->quirks = desc->openmp_quirks != 0;
+	c->D.assign(desc->diffusion_coefficients, desc->diffusion_coefficients + c->S);
+	c->lam.assign(desc->decay_rates, desc->decay_rates + c->S);
+	c->init.assign(desc->initial_conditions, desc->initial_conditions + c->S);
+	c->tuning.persistent_wgs_per_cu = 0;
+	c->tuning.yz_columns = kLinesPerTile;
+
+	auto bail = [&](int rc) {
+		g_error = c->err;
+		delete c;
+		return rc;
+	};
+
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e == hipSuccess)
+		e = init_sweep_kernels();
+	if (e == hipSuccess)
+		e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
+	if (e == hipSuccess)
+		e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+	if (e != hipSuccess)
+		return bail(c->hip_fail(e, "device setup"));
+
+	if ((e = c->field.alloc((size_t)c->V * c->S)) != hipSuccess)
+		return bail(c->hip_fail(e, "field allocation"));
+	{
+		std::vector<float> init(c->S);
+		for (int s = 0; s < c->S; s++)
+			init[s] = (float)c->init[s];
+		dvec<float> dinit;
+		if ((e = dinit.upload(init.data(), init.size(), c->stream)) != hipSuccess)
+			return bail(c->hip_fail(e, "initial conditions"));
+		if ((e = launch_fill_field(c->field.p, dinit.p, c->S, c->V, c->stream)) != hipSuccess)
+			return bail(c->hip_fail(e, "field fill"));
+		if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+			return bail(c->hip_fail(e, "field fill"));
+	}
+	for (int a = 0; a < c->dims; a++)
+	{
+		std::vector<double> b, ee, cc;
+		precompute_factors((int)c->n[a], c->S, c->voxel[a], c->dims, c->dt, c->D.data(), c->lam.data(), c->quirks, b,
+						   ee, cc);
+		if ((e = c->fb[a].upload(b.data(), b.size(), c->stream)) != hipSuccess ||
+			(e = c->fe[a].upload(ee.data(), ee.size(), c->stream)) != hipSuccess ||
+			(e = c->fc[a].upload(cc.data(), cc.size(), c->stream)) != hipSuccess)
+			return bail(c->hip_fail(e, "factor upload"));
+	}
+	if (c->quirks && c->dims >= 2)
+	{
+		std::vector<int32_t> m(c->L);
+		for (int col = 0; col < c->L; col++)
+			m[col] = openmp_coef_sub(col, c->S, c->L);
+		if ((e = c->coef_map.upload(m.data(), m.size(), c->stream)) != hipSuccess)
+			return bail(c->hip_fail(e, "coefficient map"));
+	}
+	if ((e = c->flags.alloc(4)) != hipSuccess || (e = hipMemset(c->flags.p, 0, 4 * sizeof(int32_t))) != hipSuccess)
+		return bail(c->hip_fail(e, "flags"));
+	int rc = build_tiles(c);
+	if (rc)
+		return bail(rc);
+	rc = sync_domain(c);
+	if (rc)
+		return bail(rc);
+	*out = c;
+	return BFVM_OK;
+}
+
+void bfvm_destroy(bfvm_ctx* ctx) { delete ctx; }
+
+int bfvm_set_dirichlet_faces(bfvm_ctx* c, const double* values, const uint8_t* conds)
+{
+	if (!c)
+		return BFVM_ERR_INVALID;
+	const int S = c->S;
+	std::vector<float> v(6 * S, 0.0f);
+	std::vector<uint8_t> k(6 * S, 0);
+	int mask = 0;
+	if (values && conds)
+		for (int f = 0; f < 6; f++)
+		{
+			if (f / 2 >= c->dims)
+				continue; // faces of axes beyond dims do not exist
+			for (int s = 0; s < S; s++)
+			{
+				if (conds[f * S + s])
+				{
+					if (!fp_ok(values[f * S + s]))
+						return c->fail(BFVM_ERR_INVALID, "non-finite Dirichlet value");
+					k[f * S + s] = 1;
+					mask |= 1 << f;
+				}
+				v[f * S + s] = (float)values[f * S + s];
+			}
+		}
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	TRY_HIP(c, c->face_val.upload(v.data(), v.size(), c->stream));
+	TRY_HIP(c, c->face_cond.upload(k.data(), k.size(), c->stream));
+	c->face_mask = mask;
+	return sync_domain(c);
+}
+
+int bfvm_set_dirichlet_interior(bfvm_ctx* c, int64_t count, const int64_t* voxels, const double* values,
+								const uint8_t* conds)
+{
+	if (!c || count < 0 || (count > 0 && (!voxels || !values || !conds)))
+		return c ? c->fail(BFVM_ERR_INVALID, "bad interior Dirichlet arrays") : BFVM_ERR_INVALID;
+	const int S = c->S;
+	std::vector<int64_t> lin;
+	std::vector<float> val;
+	std::vector<uint8_t> cond;
+	std::vector<int64_t> slot_of; // voxel -> entry, to merge repeats
+	for (int64_t k = 0; k < count; k++)
+	{
+		int64_t idx[3] = { 0, 0, 0 };
+		for (int a = 0; a < c->dims; a++)
+		{
+			idx[a] = voxels[k * c->dims + a];
+			if (idx[a] < 0 || idx[a] >= c->n[a])
+				return c->fail(BFVM_ERR_INVALID, "interior Dirichlet voxel outside the mesh");
+		}
+		const int64_t v = idx[0] + c->n[0] * (idx[1] + c->n[1] * idx[2]);
+		int64_t e = -1;
+		for (size_t q = 0; q < lin.size(); q++)
+			if (lin[q] == v)
+				e = (int64_t)q;
+		if (e < 0)
+		{
+			e = (int64_t)lin.size();
+			lin.push_back(v);
+			val.resize(val.size() + S, 0.0f);
+			cond.resize(cond.size() + S, 0);
+		}
+		for (int s = 0; s < S; s++)
+			if (conds[k * S + s])
+			{
+				if (!fp_ok(values[k * S + s]))
+					return c->fail(BFVM_ERR_INVALID, "non-finite Dirichlet value");
+				val[e * S + s] = (float)values[k * S + s];
+				cond[e * S + s] = 1;
+			}
+	}
+	(void)slot_of;
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	c->dir_count = (int64_t)lin.size();
+	TRY_HIP(c, c->dir_vox.upload(lin.data(), lin.size(), c->stream));
+	TRY_HIP(c, c->dir_val.upload(val.data(), val.size(), c->stream));
+	TRY_HIP(c, c->dir_cond.upload(cond.data(), cond.size(), c->stream));
+	auto r = refs_for(c->dir_count);
+	TRY_HIP(c, c->dir_refs.upload(r.data(), r.size(), c->stream));
+	return sync_domain(c);
+}
+
+int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const double* supply, const double* uptake,
+				  const double* target)
+{
+	if (!c || count < 0 || (count > 0 && (!voxels || !supply || !uptake || !target)))
+		return c ? c->fail(BFVM_ERR_INVALID, "bad bulk arrays") : BFVM_ERR_INVALID;
+	const int S = c->S;
+	std::vector<int64_t> vox(voxels, voxels + count);
+	std::vector<double> a((size_t)count * S), b((size_t)count * S);
+	{
+		std::vector<int64_t> sorted = vox;
+		std::sort(sorted.begin(), sorted.end());
+		if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+			return c->fail(BFVM_ERR_INVALID, "bulk voxel listed twice");
+		if (count && (sorted.front() < 0 || sorted.back() >= c->V))
+			return c->fail(BFVM_ERR_INVALID, "bulk voxel outside the mesh");
+	}
+	for (int64_t k = 0; k < count * S; k++)
+	{
+		if (!fp_ok(supply[k]) || !fp_ok(uptake[k]) || !fp_ok(target[k]))
+			return c->fail(BFVM_ERR_INVALID, "non-finite bulk rate");
+		// bulk_solver.cpp:29: D = (D + time_step*S*T) / (1 + time_step*(U+S))
+		a[k] = c->dt * supply[k] * target[k];
+		b[k] = 1 + c->dt * (uptake[k] + supply[k]);
+	}
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	c->bulk_count = count;
+	TRY_HIP(c, c->bulk_vox.upload(vox.data(), vox.size(), c->stream));
+	TRY_HIP(c, c->bulk_a.upload(a.data(), a.size(), c->stream));
+	TRY_HIP(c, c->bulk_b.upload(b.data(), b.size(), c->stream));
+	auto r = refs_for(count);
+	TRY_HIP(c, c->bulk_refs.upload(r.data(), r.size(), c->stream));
+	return sync_domain(c);
+}
+
+int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const double* secretion,
+					const double* saturation, const double* uptake, const double* net_export,
+					const double* internalized, const double* fraction_released, const double* volumes,
+					const uint8_t* is_active)
+{
+	if (!c || n < 0 || n > 2147483647LL)
+		return c ? c->fail(BFVM_ERR_INVALID, "bad agent count") : BFVM_ERR_INVALID;
+	if (n > 0 && (!positions || !secretion || !saturation || !uptake || !net_export || !fraction_released ||
+				  !volumes || !is_active))
+		return c->fail(BFVM_ERR_INVALID, "null agent array");
+	const size_t nS = (size_t)n * c->S;
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	c->n_agents = n;
+	c->h_positions.assign(positions, positions + (size_t)n * c->dims);
+	TRY_HIP(c, c->positions.upload(positions, (size_t)n * c->dims, c->stream));
+	TRY_HIP(c, c->secretion.upload(secretion, nS, c->stream));
+	TRY_HIP(c, c->saturation.upload(saturation, nS, c->stream));
+	TRY_HIP(c, c->uptake.upload(uptake, nS, c->stream));
+	TRY_HIP(c, c->net_export.upload(net_export, nS, c->stream));
+	if (internalized)
+		TRY_HIP(c, c->internalized_d.upload(internalized, nS, c->stream));
+	else
+	{
+		TRY_HIP(c, c->internalized_d.alloc(nS));
+		if (nS)
+			TRY_HIP(c, hipMemset(c->internalized_d.p, 0, nS * sizeof(double)));
+	}
+	TRY_HIP(c, c->fraction_released.upload(fraction_released, nS, c->stream));
+	TRY_HIP(c, c->volumes.upload(volumes, (size_t)n, c->stream));
+	TRY_HIP(c, c->is_active.upload(is_active, (size_t)n, c->stream));
+	TRY_HIP(c, c->cell_voxel.alloc(n));
+	TRY_HIP(c, c->owner_of.alloc(n));
+	TRY_HIP(c, c->owners.alloc(n));
+	TRY_HIP(c, c->num.alloc(nS));
+	TRY_HIP(c, c->den.alloc(nS));
+	TRY_HIP(c, c->fac.alloc(nS));
+	TRY_HIP(c, c->rnum.alloc(nS));
+	TRY_HIP(c, c->rden.alloc(nS));
+	TRY_HIP(c, c->rfac.alloc(nS));
+	TRY_HIP(c, c->dnew.alloc(nS));
+	auto r = refs_for(n);
+	TRY_HIP(c, c->agent_refs.upload(r.data(), r.size(), c->stream));
+	c->recompute_pending = true;
+	c->n_owners = 0;
+	return sync_domain(c);
+}
+
+int bfvm_recompute_positional_data(bfvm_ctx* c)
+{
+	if (!c)
+		return BFVM_ERR_INVALID;
+	c->recompute_pending = true;
+	c->version++;
+	return BFVM_OK;
+}
+
+int bfvm_step(bfvm_ctx* c, int64_t iterations)
+{
+	if (!c || iterations < 0)
+		return BFVM_ERR_INVALID;
+	for (int64_t it = 0; it < iterations; it++)
+	{
+		if (c->recompute_pending)
+		{
+			int rc = recompute(c);
+			if (rc)
+				return rc;
+		}
+		int rc = enqueue_step(c, nullptr);
+		if (rc)
+			return rc;
+	}
+	return BFVM_OK;
+}
+
+int bfvm_step_graph(bfvm_ctx* c, int64_t iterations, int64_t graph_steps)
+{
+	if (!c || iterations < 0 || graph_steps < 1)
+		return BFVM_ERR_INVALID;
+	if (iterations == 0)
+		return BFVM_OK;
+	if (c->recompute_pending)
+	{
+		int rc = recompute(c);
+		if (rc)
+			return rc;
+	}
+	if (!c->gexec || c->graph_version != c->version || c->graph_steps != graph_steps)
+	{
+		c->drop_graph();
+		TRY_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+		int rc = BFVM_OK;
+		for (int64_t k = 0; k < graph_steps && rc == BFVM_OK; k++)
+			rc = enqueue_step(c, nullptr);
+		hipGraph_t g = nullptr;
+		hipError_t e = hipStreamEndCapture(c->stream, &g);
+		if (rc)
+		{
+			if (g)
+				(void)hipGraphDestroy(g);
+			return rc;
+		}
+		if (e != hipSuccess)
+			return c->hip_fail(e, "hipStreamEndCapture");
+		c->graph = g;
+		TRY_HIP(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+		c->graph_steps = graph_steps;
+		c->graph_version = c->version;
+	}
+	const int64_t full = iterations / graph_steps;
+	for (int64_t k = 0; k < full; k++)
+		TRY_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+	for (int64_t k = 0; k < iterations - full * graph_steps; k++)
+	{
+		int rc = enqueue_step(c, nullptr);
+		if (rc)
+			return rc;
+	}
+	return BFVM_OK;
+}
+
+int bfvm_sync(bfvm_ctx* c)
+{
+	if (!c)
+		return BFVM_ERR_INVALID;
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_download_field_f32(bfvm_ctx* c, float* host)
+{
+	if (!c || !host)
+		return BFVM_ERR_INVALID;
+	TRY_HIP(c, hipMemcpyAsync(host, c->field.p, c->field.n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_upload_field_f32(bfvm_ctx* c, const float* host)
+{
+	if (!c || !host)
+		return BFVM_ERR_INVALID;
+	TRY_HIP(c, hipMemcpyAsync(c->field.p, host, c->field.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_download_field_f64(bfvm_ctx* c, double* host)
+{
+	if (!c || !host)
+		return BFVM_ERR_INVALID;
+	std::vector<float> tmp(c->field.n);
+	int rc = bfvm_download_field_f32(c, tmp.data());
+	if (rc)
+		return rc;
+	for (size_t k = 0; k < tmp.size(); k++)
+		host[k] = (double)tmp[k];
+	return BFVM_OK;
+}
+
+int bfvm_upload_field_f64(bfvm_ctx* c, const double* host)
+{
+	if (!c || !host)
+		return BFVM_ERR_INVALID;
+	std::vector<float> tmp(c->field.n);
+	for (size_t k = 0; k < tmp.size(); k++)
+		tmp[k] = (float)host[k];
+	return bfvm_upload_field_f32(c, tmp.data());
+}
+
+int bfvm_download_internalized(bfvm_ctx* c, double* host)
+{
+	if (!c || (!host && c->n_agents))
+		return BFVM_ERR_INVALID;
+	if (c->n_agents == 0)
+		return BFVM_OK;
+	TRY_HIP(c, hipMemcpyAsync(host, c->internalized_d.p, (size_t)c->n_agents * c->S * sizeof(double),
+							  hipMemcpyDeviceToHost, c->stream));
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_release_internalized(bfvm_ctx* c, int64_t agent)
+{
+	if (!c)
+		return BFVM_ERR_INVALID;
+	if (!c->internalized)
+		return BFVM_OK; // cell_solver.cpp:353-354
+	if (agent < 0 || agent >= c->n_agents)
+		return c->fail(BFVM_ERR_INVALID, "agent index out of range");
+	int64_t v;
+	if (!voxel_of(c, c->h_positions.data() + agent * c->dims, &v))
+		return c->fail(BFVM_ERR_OUT_OF_GRID, "agent outside the mesh");
+	TRY_HIP(c, launch_release(c->ddom.p, agent, v, c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_total_mass(bfvm_ctx* c, double* out)
+{
+	if (!c || !out)
+		return BFVM_ERR_INVALID;
+	TRY_HIP(c, c->mass_partial.alloc((size_t)mass_blocks(c->V) * c->S));
+	TRY_HIP(c, c->mass_out.alloc(c->S));
+	TRY_HIP(c, launch_mass(c->field.p, c->S, c->V, c->mass_partial.p, c->mass_out.p,
+						   (double)(c->voxel[0] * c->voxel[1] * c->voxel[2]), c->stream));
+	TRY_HIP(c, hipMemcpyAsync(out, c->mass_out.p, c->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_cell_sums(bfvm_ctx* c, int64_t* n_owners, int64_t* owners, double* num, double* den, double* fac,
+				   int32_t* is_conflict)
+{
+	if (!c || !n_owners)
+		return BFVM_ERR_INVALID;
+	if (c->recompute_pending)
+	{
+		int rc = recompute(c);
+		if (rc)
+			return rc;
+	}
+	*n_owners = c->n_owners;
+	if (is_conflict)
+		*is_conflict = c->conflict ? 1 : 0;
+	if (!owners)
+		return BFVM_OK;
+	const int S = c->S;
+	const int64_t n = c->n_agents;
+	std::vector<int32_t> o((size_t)c->n_owners);
+	std::vector<double> rn((size_t)n * S), rd((size_t)n * S), rf((size_t)n * S);
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	if (c->n_owners)
+		TRY_HIP(c, hipMemcpy(o.data(), c->owners.p, o.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (n)
+	{
+		TRY_HIP(c, hipMemcpy(rn.data(), c->rnum.p, rn.size() * sizeof(double), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpy(rd.data(), c->rden.p, rd.size() * sizeof(double), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpy(rf.data(), c->rfac.p, rf.size() * sizeof(double), hipMemcpyDeviceToHost));
+	}
+	std::sort(o.begin(), o.end());
+	for (size_t k = 0; k < o.size(); k++)
+	{
+		owners[k] = o[k];
+		for (int s = 0; s < S; s++)
+		{
+			if (num)
+				num[k * S + s] = rn[(size_t)o[k] * S + s];
+			if (den)
+				den[k * S + s] = rd[(size_t)o[k] * S + s];
+			if (fac)
+				fac[k * S + s] = rf[(size_t)o[k] * S + s];
+		}
+	}
+	return BFVM_OK;
+}
+
+int bfvm_field_device_ptr(bfvm_ctx* c, void** ptr)
+{
+	if (!c || !ptr)
+		return BFVM_ERR_INVALID;
+	*ptr = c->field.p;
+	return BFVM_OK;
+}
+
+int bfvm_stream(bfvm_ctx* c, void** stream)
+{
+	if (!c || !stream)
+		return BFVM_ERR_INVALID;
+	*stream = (void*)c->stream;
+	return BFVM_OK;
+}
+
+int bfvm_time_stages(bfvm_ctx* c, int64_t steps, double* out_us)
+{
+	if (!c || !out_us || steps < 1)
+		return BFVM_ERR_INVALID;
+	if (c->recompute_pending)
+	{
+		int rc = recompute(c);
+		if (rc)
+			return rc;
+	}
+	hipEvent_t ev[7];
+	for (auto& e : ev)
+		TRY_HIP(c, hipEventCreate(&e));
+	double acc[6] = { 0, 0, 0, 0, 0, 0 };
+	int rc = BFVM_OK;
+	for (int64_t k = 0; k < steps && rc == BFVM_OK; k++)
+	{
+		rc = enqueue_step(c, ev);
+		if (rc)
+			break;
+		hipError_t e = hipEventSynchronize(ev[6]);
+		if (e != hipSuccess)
+		{
+			rc = c->hip_fail(e, "hipEventSynchronize");
+			break;
+		}
+		for (int s = 0; s < 6; s++)
+		{
+			float ms = 0;
+			(void)hipEventElapsedTime(&ms, ev[s], ev[s + 1]);
+			acc[s] += ms * 1000.0;
+		}
+	}
+	for (auto& e : ev)
+		(void)hipEventDestroy(e);
+	if (rc)
+		return rc;
+	for (int s = 0; s < 6; s++)
+		out_us[s] = acc[s] / (double)steps;
+	return BFVM_OK;
+}
+
+int bfvm_get_tuning(const bfvm_ctx* c, bfvm_tuning* out)
+{
+	if (!c || !out)
+		return BFVM_ERR_INVALID;
+	*out = c->tuning;
+	return BFVM_OK;
+}
+
+int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
+{
+	if (!c || !t)
+		return BFVM_ERR_INVALID;
+	if (t->persistent_wgs_per_cu < 0 || t->persistent_wgs_per_cu > 64)
+		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [0, 64]");
+	if (t->yz_columns != kLinesPerTile)
+		return c->fail(BFVM_ERR_UNSUPPORTED, "yz_columns must be 64");
+	c->tuning = *t;
+	c->version++;
+	return BFVM_OK;
+}
+
+} // extern "C"

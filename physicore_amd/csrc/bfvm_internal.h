@@ -1,0 +1,136 @@
+// Internal interface between the kernels (bfvm_kernels.hip) and the context
+// logic (bfvm_context.hip) of libbfvm_hip.so.  Not installed, not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bfvm {
+
+constexpr int kSweepThreads = 256; // workgroup of every sweep kernel (4 wave64s)
+constexpr int kLinesPerTile = 64;  // lines solved per tile: one per lane of one wave
+constexpr int kFaces = 6;		   // x-min, x-max, y-min, y-max, z-min, z-max
+
+// Everything a kernel needs to know about one domain; lives in device memory
+// (one entry per domain of a launch).
+struct DomainDev
+{
+	float* field; // ((z*ny + y)*nx + x)*S + s
+	int32_t nx, ny, nz, S, dims, L; // L = nx*S, one row
+	int32_t face_mask;				// bit f: face f has a condition for some substrate
+	int32_t pad0;
+	double dt, voxel_volume;
+	int64_t bbox_min[3];
+	int64_t voxel[3];
+
+	// FP64 Thomas factors per axis: b[n*S], e[(n-1)*S], c[S]
+	const double* fb[3];
+	const double* fe[3];
+	const double* fc[3];
+	const int32_t* coef_map; // [L] coefficient substrate of a y/z column, or null (c % S)
+
+	const float* face_val;	  // [6*S]
+	const uint8_t* face_cond; // [6*S]
+
+	// interior Dirichlet: linear voxel, values, conditions
+	const int64_t* dir_vox;
+	const float* dir_val;
+	const uint8_t* dir_cond;
+	// bulk: linear voxel, a = dt*S*T, b = 1 + dt*(U+S) (FP64)
+	const int64_t* bulk_vox;
+	const double* bulk_a;
+	const double* bulk_b;
+
+	// agents (FP64, as the reference)
+	int64_t n_agents;
+	const double* positions;
+	const double* secretion;
+	const double* saturation;
+	const double* uptake;
+	const double* net_export;
+	double* internalized;
+	const double* fraction_released;
+	const double* volumes;
+	const uint8_t* is_active;
+	// cell stage state, rebuilt at recompute
+	int32_t* cell_voxel; // [n] linear voxel, -1 inactive
+	int32_t* owner_of;	 // [n] owner agent of the cell's voxel
+	int32_t* ballot;	 // [V] owner agent per voxel (atomic mode)
+	double* num;		 // [n*S] per-agent terms
+	double* den;
+	double* fac;
+	double* rnum; // [n*S] reduced sums, indexed by owner
+	double* rden;
+	double* rfac;
+	double* dnew; // [n*S] new density at the owner's voxel (FP64)
+	int32_t* owners; // [n] owner agents
+	int32_t* flags;	 // [0] conflict, [1] agent outside grid, [2] owner count
+	int32_t compute_internalized;
+	int32_t pad1;
+};
+
+// One tile of a sweep: 64 lines of one domain.
+//   x: a = first row (row = y + ny*z), b = rows in the tile
+//   y: a = z, b = first column;  z: a = y, b = first column
+struct TileRef
+{
+	int32_t dom, a, b, pad;
+};
+
+// An entry of a sparse per-voxel stage: (domain, index in that domain's list).
+struct SparseRef
+{
+	int32_t dom, idx;
+};
+
+// LDS bytes a sweep tile needs for line length n and S substrates.
+// LDS bytes of the tile plus the factor tables for fac_cap doubles per table.
+size_t sweep_lds_bytes(int axis, int n, int S, int L, int fac_cap);
+int x_rows_per_tile(int S);
+int x_row_pitch(int L, int S);
+
+// Raises the dynamic-LDS limit of the sweep kernels to 160 KiB (once per process).
+hipError_t init_sweep_kernels();
+
+// ---- launchers (enqueue on `stream`, return hipError_t) ----
+// fac_cap: doubles reserved per factor table in LDS (max n*S over the domains)
+hipError_t launch_sweep(int axis, bool fp64, const DomainDev* doms, const TileRef* tiles, int ntiles, int finish,
+						int fac_cap, size_t lds_bytes, int grid, hipStream_t stream);
+hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
+hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
+hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* owner_refs, int64_t n, hipStream_t stream);
+hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* cell_refs, int64_t n,
+									 hipStream_t stream);
+hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
+hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);
+hipError_t launch_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream);
+hipError_t launch_release(const DomainDev* dom, int64_t agent, int64_t voxel, hipStream_t stream);
+
+// cell recompute (single domain)
+hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream);
+hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stream);
+hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream);
+// sorted mode: keys/vals are scratch of n entries (sorted in place by radix_sort_pairs)
+hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
+								  hipStream_t stream);
+hipError_t launch_cells_heads(const uint32_t* keys, int32_t* heads, int64_t n, uint32_t V, hipStream_t stream);
+hipError_t launch_cells_segment_reduce(const DomainDev* dom, const uint32_t* keys, const uint32_t* vals,
+									   const int32_t* heads, const int32_t* head_scan, int64_t n,
+									   hipStream_t stream);
+
+// LSD radix sort of (key, value) pairs on the low `bits` bits of the keys; stable.
+// tmp_keys/tmp_vals: n entries; hist: radix_hist_entries(n) ints.
+size_t radix_hist_entries(int64_t n);
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, int32_t* hist,
+							int32_t* scan_tmp, int64_t n, int bits, hipStream_t stream);
+// Exclusive scan of n int32 in place; tmp needs scan_tmp_entries(n) ints.
+size_t scan_tmp_entries(int64_t n);
+hipError_t exclusive_scan(int32_t* data, int32_t* tmp, int64_t n, hipStream_t stream);
+
+// FP64 per-substrate sum of the field (fixed order); partial: S*mass_blocks(nvox) doubles.
+int mass_blocks(int64_t nvox);
+hipError_t launch_mass(const float* field, int S, int64_t nvox, double* partial, double* out, double scale,
+					   hipStream_t stream);
+
+} // namespace bfvm

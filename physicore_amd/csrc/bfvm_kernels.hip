@@ -1,0 +1,926 @@
+// bfvm_kernels.hip -- CDNA4 (gfx950) kernels of the BioFVM backend.
+//
+// The step of the reference (reactions-diffusion/biofvm/kernels/openmp_solver/src/openmp_solver.cpp:23-40)
+// is diffusion (three Thomas sweeps, LOD) -> Dirichlet -> bulk -> cells.  Each
+// sweep is HBM-bound (about 0.6 flop per byte), so every kernel here is written
+// for bandwidth: a tile of 64 lines is staged through LDS with coalesced loads,
+// solved one line per lane, and written back with coalesced stores.  Face
+// Dirichlet conditions are applied while the last sweep writes its tile.
+#include "bfvm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace bfvm {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float apply_faces(const DomainDev& d, int x, int y, int z, int s, float v)
+{
+	// Later faces win: x-min, x-max, y-min, y-max, z-min, z-max
+	// (openmp_solver/src/dirichlet_solver.cpp:59-83), so test from the last.
+	const int m = d.face_mask;
+	const int S = d.S;
+	if ((m & 32) && z == d.nz - 1 && d.face_cond[5 * S + s])
+		return d.face_val[5 * S + s];
+	if ((m & 16) && z == 0 && d.face_cond[4 * S + s])
+		return d.face_val[4 * S + s];
+	if ((m & 8) && y == d.ny - 1 && d.face_cond[3 * S + s])
+		return d.face_val[3 * S + s];
+	if ((m & 4) && y == 0 && d.face_cond[2 * S + s])
+		return d.face_val[2 * S + s];
+	if ((m & 2) && x == d.nx - 1 && d.face_cond[1 * S + s])
+		return d.face_val[1 * S + s];
+	if ((m & 1) && x == 0 && d.face_cond[0 * S + s])
+		return d.face_val[0 * S + s];
+	return v;
+}
+
+// Thomas recurrence of one line held in LDS (openmp_solver/src/diffusion_solver.h:11-33):
+//   forward   d_i += e_{i-1} d_{i-1};  d_{n-1} *= b_{n-1}
+//   backward  d_i = (d_i + c d_{i+1}) b_i
+// ACC is the arithmetic type; the LDS copy of the line stays FP32 like the field.
+template <typename ACC>
+__device__ __forceinline__ void solve_line(float* line, int stride, int n, int S, int cs, const double* fb,
+										   const double* fe, double c)
+{
+	ACC prev = (ACC)line[0];
+	for (int i = 1; i < n; i++)
+	{
+		float* p = line + i * stride;
+		const ACC v = (ACC)(*p) + (ACC)fe[(i - 1) * S + cs] * prev;
+		*p = (float)v;
+		prev = v;
+	}
+	ACC nxt = prev * (ACC)fb[(n - 1) * S + cs];
+	line[(n - 1) * stride] = (float)nxt;
+	const ACC cc = (ACC)c;
+	for (int i = n - 2; i >= 0; i--)
+	{
+		float* p = line + i * stride;
+		nxt = ((ACC)(*p) + cc * nxt) * (ACC)fb[i * S + cs];
+		*p = (float)nxt;
+	}
+}
+
+// One kernel for the three axes.  LDS: [b: fac_cap][e: fac_cap][c: 64] doubles,
+// then the FP32 tile.  A workgroup loops over tiles (persistent when the grid
+// is smaller than the tile count) and reloads the factors only when the
+// domain changes.
+template <int AXIS, typename ACC>
+__global__ __launch_bounds__(kSweepThreads) void sweep_kernel(const DomainDev* __restrict__ doms,
+															  const TileRef* __restrict__ tiles, int ntiles,
+															  int finish, int fac_cap)
+{
+	extern __shared__ double lds[];
+	double* fb = lds;
+	double* fe = lds + fac_cap;
+	double* fc = lds + 2 * fac_cap;
+	float* tile = reinterpret_cast<float*>(lds + 2 * fac_cap + 64);
+
+	const int tid = threadIdx.x;
+	int cur = -1;
+
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+	{
+		const TileRef tr = tiles[t];
+		const DomainDev& d = doms[tr.dom];
+		const int S = d.S;
+		const int n = AXIS == 0 ? d.nx : (AXIS == 1 ? d.ny : d.nz);
+
+		if (tr.dom != cur)
+		{
+			const double* gb = d.fb[AXIS];
+			const double* ge = d.fe[AXIS];
+			for (int k = tid; k < n * S; k += kSweepThreads)
+				fb[k] = gb[k];
+			for (int k = tid; k < (n - 1) * S; k += kSweepThreads)
+				fe[k] = ge[k];
+			for (int k = tid; k < S; k += kSweepThreads)
+				fc[k] = d.fc[AXIS][k];
+			cur = tr.dom;
+		}
+
+		const int L = d.L;
+		float* __restrict__ field = d.field;
+
+		if constexpr (AXIS == 0)
+		{
+			// tile = `rows` consecutive rows (contiguous in HBM), LDS row pitch Lp
+			// with Lp = S (mod 64) so the 64 lanes (r, s) hit distinct banks.
+			const int rows = tr.b;
+			const int Lp = L + ((S - L % 64) % 64 + 64) % 64;
+			const float* g = field + (size_t)tr.a * L;
+			const int total = rows * L;
+			{
+				int r = tid / L, j = tid - (tid / L) * L;
+				for (int idx = tid; idx < total; idx += kSweepThreads)
+				{
+					tile[r * Lp + j] = g[idx];
+					j += kSweepThreads;
+					while (j >= L)
+					{
+						j -= L;
+						r++;
+					}
+				}
+			}
+			__syncthreads();
+			if (tid < rows * S)
+			{
+				const int r = tid / S, s = tid - (tid / S) * S;
+				solve_line<ACC>(tile + r * Lp + s, S, n, S, s, fb, fe, fc[s]);
+			}
+			__syncthreads();
+			{
+				float* go = field + (size_t)tr.a * L;
+				int r = tid / L, j = tid - (tid / L) * L;
+				const bool faces = finish && d.face_mask;
+				for (int idx = tid; idx < total; idx += kSweepThreads)
+				{
+					float v = tile[r * Lp + j];
+					if (faces)
+					{
+						const int row = tr.a + r;
+						const int x = j / S, s = j - (j / S) * S;
+						v = apply_faces(d, x, row % d.ny, row / d.ny, s, v);
+					}
+					go[idx] = v;
+					j += kSweepThreads;
+					while (j >= L)
+					{
+						j -= L;
+						r++;
+					}
+				}
+			}
+		}
+		else
+		{
+			// tile = n rows x 64 consecutive columns (x*S + s) of one plane,
+			// line i of column j at LDS [i*64 + j].
+			const int c0 = tr.b;
+			const int w = min(kLinesPerTile, L - c0);
+			const size_t line_stride = AXIS == 1 ? (size_t)L : (size_t)d.ny * L;
+			const size_t base = AXIS == 1 ? (size_t)tr.a * d.ny * L : (size_t)tr.a * L;
+			const int j = tid & (kLinesPerTile - 1);
+			const int k0 = tid / kLinesPerTile;
+			const float* g = field + base + c0;
+			if (j < w)
+				for (int k = k0; k < n; k += kSweepThreads / kLinesPerTile)
+					tile[k * kLinesPerTile + j] = g[k * line_stride + j];
+			__syncthreads();
+			if (tid < w)
+			{
+				const int c = c0 + tid;
+				const int cs = d.coef_map ? d.coef_map[c] : c % S;
+				solve_line<ACC>(tile + tid, kLinesPerTile, n, S, cs, fb, fe, fc[cs]);
+			}
+			__syncthreads();
+			if (j < w)
+			{
+				float* go = field + base + c0;
+				const bool faces = finish && d.face_mask;
+				const int c = c0 + j;
+				const int x = c / S, s = c - (c / S) * S;
+				for (int k = k0; k < n; k += kSweepThreads / kLinesPerTile)
+				{
+					float v = tile[k * kLinesPerTile + j];
+					if (faces)
+						v = AXIS == 1 ? apply_faces(d, x, k, tr.a, s, v) : apply_faces(d, x, tr.a, k, s, v);
+					go[k * line_stride + j] = v;
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Interior Dirichlet voxels (dirichlet_solver.cpp:22-42).
+__global__ void dirichlet_interior_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
+										  int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const SparseRef r = refs[k];
+	const DomainDev& d = doms[r.dom];
+	const int S = d.S;
+	float* D = d.field + d.dir_vox[r.idx] * S;
+	for (int s = 0; s < S; s++)
+		if (d.dir_cond[(int64_t)r.idx * S + s])
+			D[s] = d.dir_val[(int64_t)r.idx * S + s];
+}
+
+// Bulk supply/uptake (bulk_solver.cpp:12-41): D = (D + dt*S*T) / (1 + dt*(U+S)),
+// with a = dt*S*T and b = 1 + dt*(U+S) precomputed in FP64 on the host.
+__global__ void bulk_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs, int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const SparseRef r = refs[k];
+	const DomainDev& d = doms[r.dom];
+	const int S = d.S;
+	float* D = d.field + d.bulk_vox[r.idx] * S;
+	const double* a = d.bulk_a + (int64_t)r.idx * S;
+	const double* b = d.bulk_b + (int64_t)r.idx * S;
+	for (int s = 0; s < S; s++)
+		D[s] = (float)(((double)D[s] + a[s]) / b[s]);
+}
+
+// ---------------------------------------------------------------- cells
+
+#pragma clang fp contract(off)
+
+// compute_intermediates + the clearing half of clear_ballots
+// (openmp_solver/src/cell_solver.cpp:22-77); voxel of the agent as
+// cartesian_mesh::voxel_position (reactions-diffusion/biofvm/src/mesh.cpp:37-61).
+__global__ void cells_bin_kernel(const DomainDev* __restrict__ dp, int64_t n, int clear_ballots)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const DomainDev& d = *dp;
+	if (!d.is_active[i])
+	{
+		d.cell_voxel[i] = -1;
+		d.owner_of[i] = -1;
+		return;
+	}
+	const int64_t nn[3] = { d.nx, d.ny, d.nz };
+	int64_t idx[3] = { 0, 0, 0 };
+	for (int a = 0; a < d.dims; a++)
+	{
+		const double q = (d.positions[i * d.dims + a] - (double)d.bbox_min[a]) / (double)d.voxel[a];
+		if (!(q >= 0.0) || q >= (double)nn[a])
+		{
+			d.flags[1] = 1;
+			d.cell_voxel[i] = -1;
+			d.owner_of[i] = -1;
+			return;
+		}
+		idx[a] = (int64_t)q;
+	}
+	const int64_t v = idx[0] + nn[0] * (idx[1] + nn[1] * idx[2]);
+	d.cell_voxel[i] = (int32_t)v;
+	const int S = d.S;
+	const double V = d.voxel_volume, dt = d.dt, vol = d.volumes[i];
+	for (int s = 0; s < S; s++)
+	{
+		const int64_t k = i * S + s;
+		d.num[k] = d.secretion[k] * d.saturation[k] * dt * vol / V;
+		d.den[k] = (d.uptake[k] + d.secretion[k]) * dt * vol / V;
+		d.fac[k] = d.net_export[k] * dt / V;
+		d.rnum[k] = 0;
+		d.rden[k] = 0;
+		d.rfac[k] = 0;
+	}
+	if (clear_ballots)
+		d.ballot[v] = -1;
+}
+
+// ballot_and_sum (cell_solver.cpp:79-129) with wavefront aggregation: lanes
+// holding the same voxel in a run of consecutive lanes elect one CAS and
+// reduce their terms with a segmented scan, so one FP64 atomic per run and
+// substrate reaches memory.
+__global__ __launch_bounds__(256) void cells_ballot_kernel(const DomainDev* __restrict__ dp, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const DomainDev& d = *dp;
+	const int lane = threadIdx.x & (kWave - 1);
+	const int key = (i < n) ? d.cell_voxel[i] : -1;
+	const int prev_key = __shfl_up(key, 1);
+	const bool head = key < 0 || lane == 0 || prev_key != key;
+	int hp = head ? lane : 0;
+	for (int off = 1; off < kWave; off <<= 1)
+	{
+		const int o = __shfl_up(hp, off);
+		if (lane >= off)
+			hp = max(hp, o);
+	}
+	const int seg = hp; // first lane of my run
+	const int next_seg = __shfl_down(hp, 1);
+	const bool tail = lane == kWave - 1 || next_seg != seg;
+
+	int owner = -1;
+	int won = 0;
+	if (key >= 0 && head)
+	{
+		const int old = atomicCAS(&d.ballot[key], -1, (int)i);
+		owner = old == -1 ? (int)i : old;
+		won = old == -1;
+		if (!won)
+			d.flags[0] = 1;
+	}
+	owner = __shfl(owner, seg);
+	won = __shfl(won, seg);
+	if (key >= 0)
+	{
+		d.owner_of[i] = owner;
+		if (!head)
+			d.flags[0] = 1;
+	}
+	const int S = d.S;
+	for (int s = 0; s < S; s++)
+	{
+		double vn = 0, vd = 0, vf = 0;
+		if (key >= 0)
+		{
+			vn = d.num[i * S + s];
+			vd = d.den[i * S + s] + ((head && won) ? 1.0 : 0.0);
+			vf = d.fac[i * S + s];
+		}
+		for (int off = 1; off < kWave; off <<= 1)
+		{
+			const double on = __shfl_up(vn, off), od = __shfl_up(vd, off), of = __shfl_up(vf, off);
+			if (lane - off >= seg)
+			{
+				vn += on;
+				vd += od;
+				vf += of;
+			}
+		}
+		if (key >= 0 && tail)
+		{
+			atomicAdd(&d.rnum[(int64_t)owner * S + s], vn);
+			atomicAdd(&d.rden[(int64_t)owner * S + s], vd);
+			atomicAdd(&d.rfac[(int64_t)owner * S + s], vf);
+		}
+	}
+}
+
+__global__ void cells_collect_owners_kernel(const DomainDev* __restrict__ dp, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const DomainDev& d = *dp;
+	if (d.cell_voxel[i] >= 0 && d.owner_of[i] == (int)i)
+		d.owners[atomicAdd(&d.flags[2], 1)] = (int)i;
+}
+
+__global__ void cells_make_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const DomainDev& d = *dp;
+	const int v = d.cell_voxel[i];
+	const uint32_t V = (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz;
+	keys[i] = v >= 0 ? (uint32_t)v : V;
+	vals[i] = (uint32_t)i;
+}
+
+__global__ void cells_heads_kernel(const uint32_t* __restrict__ keys, int32_t* heads, int64_t n, uint32_t V)
+{
+	const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= n)
+		return;
+	const uint32_t k = keys[p];
+	heads[p] = (k < V && (p == 0 || keys[p - 1] != k)) ? 1 : 0;
+}
+
+// Sorted reduce: one thread per voxel segment sums its agents in ascending
+// agent order, the owner (lowest index) first with its den + 1 -- the order
+// in which a sequential run of ballot_and_sum adds them.  Bit-reproducible.
+__global__ void cells_segment_reduce_kernel(const DomainDev* __restrict__ dp, const uint32_t* __restrict__ keys,
+											const uint32_t* __restrict__ vals, const int32_t* __restrict__ heads,
+											const int32_t* __restrict__ head_scan, int64_t n)
+{
+	const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= n)
+		return;
+	const DomainDev& d = *dp;
+	if (p == n - 1)
+		d.flags[2] = head_scan[p] + heads[p];
+	if (!heads[p])
+		return;
+	const uint32_t key = keys[p];
+	const int owner = (int)vals[p];
+	d.owners[head_scan[p]] = owner;
+	const int S = d.S;
+	int64_t q1 = p + 1;
+	while (q1 < n && keys[q1] == key)
+		q1++;
+	if (q1 - p > 1)
+		d.flags[0] = 1;
+	for (int64_t q = p; q < q1; q++)
+		d.owner_of[vals[q]] = owner;
+	for (int s = 0; s < S; s++)
+	{
+		double sn = d.num[(int64_t)owner * S + s];
+		double sd = d.den[(int64_t)owner * S + s] + 1;
+		double sf = d.fac[(int64_t)owner * S + s];
+		for (int64_t q = p + 1; q < q1; q++)
+		{
+			const int64_t c = vals[q];
+			sn += d.num[c * S + s];
+			sd += d.den[c * S + s];
+			sf += d.fac[c * S + s];
+		}
+		d.rnum[(int64_t)owner * S + s] = sn;
+		d.rden[(int64_t)owner * S + s] = sd;
+		d.rfac[(int64_t)owner * S + s] = sf;
+	}
+}
+
+// compute_result, owner part (cell_solver.cpp:187-227, compute_densities :146-163,
+// compute_fused :165-185): D = (D + sum num) / sum den + sum fac at each occupied
+// voxel.  Without a conflict and with tracking on, the owner's internalized
+// amount takes V*(D_old - D_new); otherwise D_new is kept in FP64 for the
+// per-agent update.
+__global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
+								   int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const SparseRef r = refs[k];
+	const DomainDev& d = doms[r.dom];
+	const int o = d.owners[r.idx];
+	if (!d.is_active[o])
+		return;
+	const int S = d.S;
+	float* D = d.field + (int64_t)d.cell_voxel[o] * S;
+	const bool fused = d.compute_internalized && !d.flags[0];
+	for (int s = 0; s < S; s++)
+	{
+		const int64_t q = (int64_t)o * S + s;
+		const double d0 = (double)D[s];
+		const double d1 = (d0 + d.rnum[q]) / d.rden[q] + d.rfac[q];
+		D[s] = (float)d1;
+		if (fused)
+			d.internalized[q] += d.voxel_volume * (d0 - d1);
+		else
+			d.dnew[q] = d1;
+	}
+}
+
+// compute_internalized (cell_solver.cpp:131-144, :229-243):
+// I -= V*(num - D_new*den + fac) for every active agent, when voxels are shared.
+__global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
+										  int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const SparseRef r = refs[k];
+	const DomainDev& d = doms[r.dom];
+	const int64_t i = r.idx;
+	if (!d.is_active[i] || d.cell_voxel[i] < 0)
+		return;
+	const int S = d.S;
+	const int64_t o = d.owner_of[i];
+	for (int s = 0; s < S; s++)
+	{
+		const int64_t q = i * S + s;
+		d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[o * S + s] * d.den[q] + d.fac[q]);
+	}
+}
+
+// release_internal (cell_solver.cpp:274-289): D += I*F/V, I = 0.
+__global__ void release_kernel(const DomainDev* __restrict__ dp, int64_t i, int64_t voxel)
+{
+	const DomainDev& d = *dp;
+	const int S = d.S;
+	for (int s = threadIdx.x; s < S; s += blockDim.x)
+	{
+		const int64_t q = i * S + s;
+		float* D = d.field + voxel * S + s;
+		*D = (float)((double)*D + d.internalized[q] * d.fraction_released[q] / d.voxel_volume);
+		d.internalized[q] = 0;
+	}
+}
+
+#pragma clang fp contract(on)
+
+// ------------------------------------------------------------- utilities
+
+__global__ void fill_field_kernel(float* __restrict__ f, const float* __restrict__ init, int S, int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n)
+		f[k] = init[k % S];
+}
+
+__global__ void f64_to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n)
+		b[k] = (float)a[k];
+}
+
+__global__ void f32_to_f64_kernel(const float* __restrict__ a, double* __restrict__ b, int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n)
+		b[k] = (double)a[k];
+}
+
+constexpr int kMassThreads = 256;
+constexpr int64_t kMassChunk = 65536;
+
+__global__ __launch_bounds__(kMassThreads) void mass_partial_kernel(const float* __restrict__ f, int S, int64_t nvox,
+																	double* __restrict__ partial)
+{
+	__shared__ double red[kMassThreads];
+	const int s = blockIdx.y;
+	const int64_t v0 = (int64_t)blockIdx.x * kMassChunk;
+	const int64_t v1 = min(v0 + kMassChunk, nvox);
+	double acc = 0;
+	for (int64_t v = v0 + threadIdx.x; v < v1; v += kMassThreads)
+		acc += (double)f[v * S + s];
+	red[threadIdx.x] = acc;
+	__syncthreads();
+	for (int w = kMassThreads / 2; w > 0; w >>= 1)
+	{
+		if ((int)threadIdx.x < w)
+			red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		partial[(int64_t)s * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void mass_final_kernel(const double* __restrict__ partial, int nb, int S, double scale,
+								  double* __restrict__ out)
+{
+	const int s = threadIdx.x;
+	if (s >= S)
+		return;
+	double acc = 0;
+	for (int b = 0; b < nb; b++)
+		acc += partial[(int64_t)s * nb + b];
+	out[s] = acc * scale;
+}
+
+// ------------------------------------------------------------------ scan
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+// exclusive scan of kScanThreads values held one per thread; returns the total
+__device__ int block_exclusive_scan(int v, int* sh)
+{
+	const int lane = threadIdx.x & (kWave - 1);
+	const int wave = threadIdx.x / kWave;
+	int incl = v;
+	for (int off = 1; off < kWave; off <<= 1)
+	{
+		const int o = __shfl_up(incl, off);
+		if (lane >= off)
+			incl += o;
+	}
+	if (lane == kWave - 1)
+		sh[wave] = incl;
+	__syncthreads();
+	int wave_off = 0, total = 0;
+	for (int w = 0; w < kScanThreads / kWave; w++)
+	{
+		if (w < wave)
+			wave_off += sh[w];
+		total += sh[w];
+	}
+	__syncthreads();
+	sh[kScanThreads / kWave] = total;
+	return wave_off + incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(int32_t* data, int32_t* block_sums, int64_t n)
+{
+	__shared__ int sh[kScanThreads / kWave + 1];
+	const int64_t base = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+	int v[kScanItems];
+	int sum = 0;
+	for (int k = 0; k < kScanItems; k++)
+	{
+		v[k] = (base + k < n) ? data[base + k] : 0;
+		sum += v[k];
+	}
+	int off = block_exclusive_scan(sum, sh);
+	for (int k = 0; k < kScanItems; k++)
+	{
+		if (base + k < n)
+			data[base + k] = off;
+		off += v[k];
+	}
+	if (threadIdx.x == kScanThreads - 1)
+		block_sums[blockIdx.x] = off;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_single_kernel(int32_t* data, int64_t n)
+{
+	__shared__ int sh[kScanThreads / kWave + 1];
+	int carry = 0;
+	for (int64_t c0 = 0; c0 < n; c0 += kScanThreads)
+	{
+		const int64_t k = c0 + threadIdx.x;
+		const int v = k < n ? data[k] : 0;
+		const int ex = block_exclusive_scan(v, sh);
+		if (k < n)
+			data[k] = carry + ex;
+		__syncthreads();
+		carry += sh[kScanThreads / kWave];
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(int32_t* data, const int32_t* block_sums, int64_t n)
+{
+	const int add = block_sums[blockIdx.x];
+	const int64_t base = (int64_t)blockIdx.x * kScanBlock;
+	for (int k = threadIdx.x; k < kScanBlock; k += kScanThreads)
+		if (base + k < n)
+			data[base + k] += add;
+}
+
+// ----------------------------------------------------------- radix sort
+
+constexpr int kRadixThreads = 256;
+constexpr int kRadixChunk = 4096; // elements per block and pass
+constexpr int kRadixBins = 256;
+
+__global__ __launch_bounds__(kRadixThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t n,
+																   int shift, int32_t* __restrict__ hist, int nblocks)
+{
+	__shared__ int h[kRadixBins];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const int64_t b0 = (int64_t)blockIdx.x * kRadixChunk;
+	const int64_t b1 = min(b0 + kRadixChunk, n);
+	for (int64_t k = b0 + threadIdx.x; k < b1; k += kRadixThreads)
+		atomicAdd(&h[(keys[k] >> shift) & 255], 1);
+	__syncthreads();
+	hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: within a block, elements keep their order (sub-tile, wave,
+// lane), so equal digits leave in input order.
+__global__ __launch_bounds__(kRadixThreads) void radix_scatter_kernel(const uint32_t* __restrict__ keys,
+																	  const uint32_t* __restrict__ vals,
+																	  uint32_t* __restrict__ okeys,
+																	  uint32_t* __restrict__ ovals, int64_t n,
+																	  int shift, const int32_t* __restrict__ offsets,
+																	  int nblocks)
+{
+	__shared__ int base[kRadixBins];
+	__shared__ int wcount[kRadixThreads / kWave][kRadixBins];
+	const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+	base[tid] = offsets[(int64_t)tid * nblocks + blockIdx.x];
+	const int64_t b0 = (int64_t)blockIdx.x * kRadixChunk;
+	const int64_t b1 = min(b0 + kRadixChunk, n);
+	for (int64_t t0 = b0; t0 < b1; t0 += kRadixThreads)
+	{
+		for (int w = 0; w < kRadixThreads / kWave; w++)
+			wcount[w][tid] = 0;
+		__syncthreads();
+		const int64_t k = t0 + tid;
+		const bool valid = k < b1;
+		const uint32_t key = valid ? keys[k] : 0u;
+		const uint32_t val = valid ? vals[k] : 0u;
+		const int digit = (key >> shift) & 255;
+		unsigned long long m = __ballot(valid);
+		for (int bit = 0; bit < 8; bit++)
+		{
+			const unsigned long long bb = __ballot((digit >> bit) & 1);
+			m &= ((digit >> bit) & 1) ? bb : ~bb;
+		}
+		const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+		const int rank = __popcll(m & below);
+		if (valid && rank == 0)
+			wcount[wave][digit] = __popcll(m);
+		__syncthreads();
+		if (valid)
+		{
+			int pre = 0;
+			for (int w = 0; w < wave; w++)
+				pre += wcount[w][digit];
+			const int pos = base[digit] + pre + rank;
+			okeys[pos] = key;
+			ovals[pos] = val;
+		}
+		__syncthreads();
+		int add = 0;
+		for (int w = 0; w < kRadixThreads / kWave; w++)
+			add += wcount[w][tid];
+		base[tid] += add;
+		__syncthreads();
+	}
+}
+
+template <typename K, typename... Args>
+hipError_t launch1d(K kernel, int64_t n, int threads, hipStream_t stream, Args... args)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int64_t blocks = (n + threads - 1) / threads;
+	hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(threads), 0, stream, args...);
+	return hipGetLastError();
+}
+
+} // namespace
+
+// ------------------------------------------------------------ launchers
+
+int x_rows_per_tile(int S) { return S >= kLinesPerTile ? 1 : kLinesPerTile / S; }
+
+int x_row_pitch(int L, int S) { return L + ((S - L % 64) % 64 + 64) % 64; }
+
+hipError_t init_sweep_kernels()
+{
+	const int max_lds = 160 * 1024;
+	const void* fns[] = { (const void*)sweep_kernel<0, double>, (const void*)sweep_kernel<1, double>,
+						  (const void*)sweep_kernel<2, double>, (const void*)sweep_kernel<0, float>,
+						  (const void*)sweep_kernel<1, float>,	(const void*)sweep_kernel<2, float> };
+	for (const void* f : fns)
+	{
+		hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+		if (e != hipSuccess)
+			return e;
+	}
+	return hipSuccess;
+}
+
+size_t sweep_lds_bytes(int axis, int n, int S, int L, int fac_cap)
+{
+	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
+	size_t tile;
+	if (axis == 0)
+		tile = (size_t)x_rows_per_tile(S) * x_row_pitch(L, S) * sizeof(float);
+	else
+		tile = (size_t)n * kLinesPerTile * sizeof(float);
+	return fac + tile;
+}
+
+hipError_t launch_sweep(int axis, bool fp64, const DomainDev* doms, const TileRef* tiles, int ntiles, int finish,
+							int fac_cap, size_t lds_bytes, int grid, hipStream_t stream)
+{
+	if (ntiles <= 0)
+		return hipSuccess;
+	if (grid <= 0 || grid > ntiles)
+		grid = ntiles;
+	const dim3 g(grid), b(kSweepThreads);
+#define BFVM_SWEEP(AX, T) \
+	hipLaunchKernelGGL((sweep_kernel<AX, T>), g, b, lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap)
+	if (fp64)
+	{
+		if (axis == 0)
+			BFVM_SWEEP(0, double);
+		else if (axis == 1)
+			BFVM_SWEEP(1, double);
+		else
+			BFVM_SWEEP(2, double);
+	}
+	else
+	{
+		if (axis == 0)
+			BFVM_SWEEP(0, float);
+		else if (axis == 1)
+			BFVM_SWEEP(1, float);
+		else
+			BFVM_SWEEP(2, float);
+	}
+#undef BFVM_SWEEP
+	return hipGetLastError();
+}
+
+hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+{
+	return launch1d(dirichlet_interior_kernel, n, 256, stream, doms, refs, n);
+}
+
+hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+{
+	return launch1d(bulk_kernel, n, 256, stream, doms, refs, n);
+}
+
+hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_apply_kernel, n, 256, stream, doms, refs, n);
+}
+
+hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_internalized_kernel, n, 256, stream, doms, refs, n);
+}
+
+hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream)
+{
+	const int64_t n = nvox * S;
+	return launch1d(fill_field_kernel, n, 256, stream, field, init, S, n);
+}
+
+hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream)
+{
+	return launch1d(f64_to_f32_kernel, n, 256, stream, src, dst, n);
+}
+
+hipError_t launch_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream)
+{
+	return launch1d(f32_to_f64_kernel, n, 256, stream, src, dst, n);
+}
+
+hipError_t launch_release(const DomainDev* dom, int64_t agent, int64_t voxel, hipStream_t stream)
+{
+	hipLaunchKernelGGL(release_kernel, dim3(1), dim3(64), 0, stream, dom, agent, voxel);
+	return hipGetLastError();
+}
+
+hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream)
+{
+	return launch1d(cells_bin_kernel, n, 256, stream, dom, n, clear_ballots ? 1 : 0);
+}
+
+hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_ballot_kernel, n, 256, stream, dom, n);
+}
+
+hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_collect_owners_kernel, n, 256, stream, dom, n);
+}
+
+hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_make_keys_kernel, n, 256, stream, dom, keys, vals, n);
+}
+
+hipError_t launch_cells_heads(const uint32_t* keys, int32_t* heads, int64_t n, uint32_t V, hipStream_t stream)
+{
+	return launch1d(cells_heads_kernel, n, 256, stream, keys, heads, n, V);
+}
+
+hipError_t launch_cells_segment_reduce(const DomainDev* dom, const uint32_t* keys, const uint32_t* vals,
+									   const int32_t* heads, const int32_t* head_scan, int64_t n,
+									   hipStream_t stream)
+{
+	return launch1d(cells_segment_reduce_kernel, n, 256, stream, dom, keys, vals, heads, head_scan, n);
+}
+
+size_t scan_tmp_entries(int64_t n) { return (size_t)((n + kScanBlock - 1) / kScanBlock) + 1; }
+
+hipError_t exclusive_scan(int32_t* data, int32_t* tmp, int64_t n, hipStream_t stream)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+	hipLaunchKernelGGL(scan_blocks_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, data, tmp, n);
+	hipLaunchKernelGGL(scan_single_kernel, dim3(1), dim3(kScanThreads), 0, stream, tmp, nb);
+	hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, data, tmp, n);
+	return hipGetLastError();
+}
+
+size_t radix_hist_entries(int64_t n)
+{
+	const int64_t nb = (n + kRadixChunk - 1) / kRadixChunk;
+	return (size_t)(nb * kRadixBins);
+}
+
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, int32_t* hist,
+							int32_t* scan_tmp, int64_t n, int bits, hipStream_t stream)
+{
+	if (n <= 1)
+		return hipSuccess;
+	const int nb = (int)((n + kRadixChunk - 1) / kRadixChunk);
+	uint32_t *ik = keys, *iv = vals, *ok = tmp_keys, *ov = tmp_vals;
+	int passes = 0;
+	for (int shift = 0; shift < bits; shift += 8, passes++)
+	{
+		hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kRadixThreads), 0, stream, ik, n, shift, hist, nb);
+		hipError_t e = exclusive_scan(hist, scan_tmp, (int64_t)nb * kRadixBins, stream);
+		if (e != hipSuccess)
+			return e;
+		hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kRadixThreads), 0, stream, ik, iv, ok, ov, n, shift,
+						   hist, nb);
+		std::swap(ik, ok);
+		std::swap(iv, ov);
+	}
+	if (passes & 1)
+	{
+		hipError_t e = hipMemcpyAsync(keys, ik, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+		if (e != hipSuccess)
+			return e;
+		e = hipMemcpyAsync(vals, iv, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+		if (e != hipSuccess)
+			return e;
+	}
+	return hipGetLastError();
+}
+
+int mass_blocks(int64_t nvox) { return (int)((nvox + kMassChunk - 1) / kMassChunk); }
+
+hipError_t launch_mass(const float* field, int S, int64_t nvox, double* partial, double* out, double scale,
+					   hipStream_t stream)
+{
+	const int nb = mass_blocks(nvox);
+	hipLaunchKernelGGL(mass_partial_kernel, dim3(nb, S), dim3(kMassThreads), 0, stream, field, S, nvox, partial);
+	hipLaunchKernelGGL(mass_final_kernel, dim3(1), dim3(64 * ((S + 63) / 64)), 0, stream, partial, nb, S, scale,
+					   out);
+	return hipGetLastError();
+}
+
+} // namespace bfvm

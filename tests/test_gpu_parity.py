@@ -1,0 +1,413 @@
+"""Parity of the HIP backend with the CPU oracle, through the C ABI.
+
+Tolerances: the field is FP32 in HBM, the oracle FP64.  One step is held to a
+relative L-infinity error per substrate of 2e-6 (a few FP32 roundings); multi-step
+runs to the 1e-5 bar of the benchmark.  Cell sums are FP64 on both sides: the
+sorted reduction must match the oracle bit for bit, the atomic one to 1e-13.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import scenarios as sc
+from gpu_helpers import gpu_domain, rel_linf, require_gpu, set_agents
+
+pytestmark = pytest.mark.gpu
+G = sc.GOLDEN
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+
+
+# ------------------------------------------------------------------ diffusion
+
+@pytest.mark.parametrize("name,dims", [("Uniform1D", 1), ("Uniform2D", 2), ("Uniform3D", 3)])
+def test_uniform_golden(name, dims):
+    t = G["diffusion"][name]
+    env = G["diffusion"][t["env"]]
+    p = sc.problem_from_env(env, dims, t["bbox_max"], t["voxel"])
+    d = gpu_domain(p, field=sc.constant_field(p, env["initial"]))
+    d.step(1)
+    f = d.field(np.float64)
+    for s, want in enumerate(t["expected_per_substrate"]):
+        assert np.abs(f[..., s] - want).max() < max(t["tol"], 4e-7 * want)
+
+
+@pytest.mark.parametrize("name,dims", [("Random1D", 1), ("Random2D", 2), ("Random3D", 3)])
+def test_random_golden(name, dims):
+    t = G["diffusion"][name]
+    env = G["diffusion"][t["env"]]
+    p = sc.problem_from_env(env, dims, t["bbox_max"], t["voxel"])
+    d = gpu_domain(p, field=sc.linear_index_field(p))
+    d.step(1)
+    got = d.field(np.float64).ravel()
+    want = np.asarray(t["expected"])
+    # FP32 storage: 1e-6 absolute, or a few ulps of the value
+    assert np.all(np.abs(got - want) <= np.maximum(t["tol"], 2.5e-7 * np.abs(want)))
+
+
+SHAPES = [
+    (3, (5, 4, 3), 1), (3, (7, 9, 11), 2), (3, (33, 17, 9), 3), (3, (64, 64, 64), 4),
+    (3, (50, 50, 50), 1), (3, (13, 70, 5), 5), (3, (40, 6, 31), 7), (3, (16, 16, 16), 8),
+    (3, (1, 9, 12), 2), (3, (9, 1, 12), 3), (3, (9, 12, 1), 2), (2, (37, 29, 1), 3),
+    (1, (101, 1, 1), 2), (3, (129, 3, 4), 1), (3, (3, 130, 5), 2), (3, (4, 5, 200), 4),
+]
+
+
+@pytest.mark.parametrize("dims,grid,S", SHAPES)
+@pytest.mark.parametrize("quirks", [False, True])
+def test_diffusion_step_matches_oracle(dims, grid, S, quirks):
+    rng = np.random.default_rng(hash((dims, grid, S)) % 2**32)
+    D = 10 ** rng.uniform(0, 5, S)
+    lam = rng.uniform(0.01, 1, S)
+    p = orc.Problem(dims, grid, (20, 20, 20), 0.01, D, lam, openmp_quirks=quirks)
+    f0 = rng.uniform(0, 1, p.shape)
+    want = p.diffuse(f0.astype(np.float32).astype(np.float64))
+    d = gpu_domain(p, field=f0)
+    d.step(1)
+    err = rel_linf(d.field(), want)
+    assert err.max() < 2e-6, err
+
+
+def test_fp32_arithmetic_mode_runs():
+    rng = np.random.default_rng(4)
+    p = orc.Problem(3, (20, 30, 40), (20, 20, 20), 0.01, [1e3, 3.0], [0.1, 0.5])
+    f0 = rng.uniform(0, 1, p.shape)
+    want = p.diffuse(f0.astype(np.float32).astype(np.float64))
+    d = gpu_domain(p, field=f0, precision="fp32")
+    d.step(1)
+    assert rel_linf(d.field(), want).max() < 1e-5
+
+
+def test_pure_diffusion_mass_balance():
+    """Neumann sweeps conserve line mass up to the decay share, so k steps divide
+    each substrate's total by (1 + dt*lambda/3)^(3k) exactly."""
+    rng = np.random.default_rng(6)
+    p = orc.Problem(3, (48, 40, 56), (20, 20, 20), 0.01, [1e5, 10.0, 1.0], [0.5, 0.01, 1.0])
+    d = gpu_domain(p, field=rng.uniform(0, 1, p.shape))
+    m0 = d.total_mass()
+    k = 50
+    d.step(k)
+    m1 = d.total_mass()
+    np.testing.assert_allclose(m1, m0 / (1 + p.dt * p.decay / 3) ** (3 * k), rtol=2e-6)
+    # the device reduction equals an FP64 host sum of the same FP32 field
+    f = d.field(np.float64)
+    np.testing.assert_allclose(m1, f.sum(axis=(0, 1, 2)) * 8000.0, rtol=1e-12)
+
+
+# ----------------------------------------------------------- Dirichlet / bulk
+
+@pytest.mark.parametrize("name,dims", [("Interior1D", 1), ("Interior2D", 2), ("Interior3D", 3)])
+def test_dirichlet_interior(name, dims):
+    t = G["dirichlet"][name]
+    env = G["dirichlet"]["default_microenv"]
+    p = sc.problem_from_env(env, dims, t["bbox_max"], t["voxel"])
+    # D = lambda = 0 makes the sweeps the identity, so a step is the Dirichlet stage
+    q = orc.Problem(dims, (p.nx, p.ny, p.nz), p.voxel, p.dt, [0.0, 0.0], [0.0, 0.0])
+    spec = orc.DirichletSpec(2, dims)
+    spec.set_interior([v[:dims] for v in t["voxels"]], [[v, 0] for v in t["values"]],
+                      [[1, 0] for _ in t["values"]])
+    d = gpu_domain(q, field=sc.constant_field(q, env["initial"]), dirichlet=spec)
+    d.step(1)
+    f = d.field(np.float64)
+    for v, val in zip(t["voxels"], t["values"]):
+        x, y, z = v
+        assert f[z, y, x, 0] == val
+    assert np.all(f[..., 1] == 1)
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_dirichlet_boundary_precedence(dims):
+    t = G["dirichlet"]["Boundary"]
+    env = G["dirichlet"]["default_microenv"]
+    p = sc.problem_from_env(env, dims, t["bbox_max"], t["voxel"])
+    q = orc.Problem(dims, (p.nx, p.ny, p.nz), p.voxel, p.dt, [0.0, 0.0], [0.0, 0.0])
+    spec = orc.DirichletSpec(2, dims)
+    for a, ax in enumerate("xyz"[:dims]):
+        lo, hi = t["face_values"][ax]
+        spec.set_face(a, "min", [lo, 42], [1, 0])
+        spec.set_face(a, "max", [hi, 42], [1, 0])
+    want = orc.dirichlet(q, sc.constant_field(q, env["initial"]), spec)
+    d = gpu_domain(q, field=sc.constant_field(q, env["initial"]), dirichlet=spec)
+    d.step(1)
+    assert np.array_equal(d.field(np.float64), want)
+
+
+def test_dirichlet_precedence_interior_over_faces_random():
+    rng = np.random.default_rng(8)
+    S = 3
+    p = orc.Problem(3, (9, 8, 7), (10, 10, 10), 0.05, [50.0, 5.0, 500.0], [0.1, 0.2, 0.3])
+    spec = orc.DirichletSpec(S, 3)
+    for a in range(3):
+        for side in ("min", "max"):
+            spec.set_face(a, side, rng.uniform(0, 9, S), rng.integers(0, 2, S))
+    vox = [(0, 0, 0), (8, 7, 6), (4, 4, 4), (0, 3, 6), (4, 4, 4)]
+    spec.set_interior(vox, rng.uniform(10, 20, (5, S)), rng.integers(0, 2, (5, S)))
+    f0 = rng.uniform(0, 1, p.shape)
+    s = orc.Solver(p, f0.astype(np.float32).astype(np.float64), dirichlet=spec)
+    s.solve(3)
+    d = gpu_domain(p, field=f0, dirichlet=spec)
+    d.step(3)
+    assert rel_linf(d.field(), s.field).max() < 3e-6
+
+
+def test_bulk_golden():
+    t = G["bulk"]["SimulateBulkSource"]
+    p = sc.problem_from_env(t, 3, t["bbox_max"], t["voxel"])
+    q = orc.Problem(3, (p.nx, p.ny, p.nz), p.voxel, p.dt, [0.0, 0.0], [0.0, 0.0])
+    sup, upt, tgt = (np.zeros(q.shape) for _ in range(3))
+    x, y, z = t["point"]
+    sup[z, y, x, 0], tgt[z, y, x, 0], upt[z, y, x, 0] = t["supply"], t["target"], t["uptake"]
+    d = gpu_domain(q, field=sc.constant_field(q, t["initial"]), bulk=(sup, upt, tgt))
+    d.step(1)
+    f = d.field(np.float64)
+    assert abs(f[z, y, x, 0] - t["expected_point"]) < t["tol_point"]
+    f[z, y, x, 0] = 10
+    assert np.all(f[..., 0] == 10) and np.all(f[..., 1] == 1)
+
+
+def test_bulk_dense_random_matches_oracle():
+    rng = np.random.default_rng(9)
+    p = orc.Problem(3, (12, 10, 8), (20, 20, 20), 0.01, [100.0, 1.0], [0.1, 0.3])
+    sup, upt, tgt = (np.where(rng.uniform(0, 1, p.shape) < 0.3, rng.uniform(0, 5, p.shape), 0)
+                     for _ in range(3))
+    f0 = rng.uniform(0, 1, p.shape)
+    s = orc.Solver(p, f0.astype(np.float32).astype(np.float64), bulk=(sup, upt, tgt))
+    s.solve(2)
+    d = gpu_domain(p, field=f0, bulk=(sup, upt, tgt))
+    d.step(2)
+    assert rel_linf(d.field(), s.field).max() < 2e-6
+
+
+# ---------------------------------------------------------------------- cells
+
+def _cells_problem(p):
+    return orc.Problem(p.dims, (p.nx, p.ny, p.nz), p.voxel, p.dt, [0.0] * p.S, [0.0] * p.S)
+
+
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+@pytest.mark.parametrize("internalized", [True, False])
+@pytest.mark.parametrize("recompute", [True, False])
+def test_cells_simple_golden(reduce, dims, internalized, recompute):
+    g = G["cells"]["Simple"]
+    p, ag = sc.simple_cells(dims)
+    q = _cells_problem(p)
+    d = gpu_domain(q, field=sc.constant_field(q, [1, 1]), agents=ag, internalized=internalized,
+                   reduce=reduce)
+    diag = [(0, 0, 0), (1, 1, 1), (2, 2, 2)]
+
+    def check(step):
+        f = d.field(np.float64)
+        I = d.internalized()
+        for k in range(3):
+            x, y, z = diag[k]
+            for sub in range(2):
+                v = f[z if dims > 2 else 0, y if dims > 1 else 0, x, sub]
+                want, tol = g[f"step{step}_density"][2 * k + sub]
+                assert abs(v - want) < max(tol, 3e-7 * abs(want))
+                if internalized:
+                    want, tol = g[f"step{step}_internalized"][2 * k + sub]
+                    assert abs(I[k, sub] - want) < max(tol, 2e-7 * abs(want))
+
+    d.step(1)
+    check(1)
+    if recompute:
+        d.recompute_positional_data()
+    d.step(1)
+    check(2)
+    d.release_internalized(0)
+    if internalized:
+        f = d.field(np.float64)
+        assert f[0, 0, 0, 0] == 1.0 and f[0, 0, 0, 1] == 1.0
+        assert np.all(d.internalized()[0] == 0)
+
+
+@pytest.mark.parametrize("builder", [sc.conflict_cells, sc.conflict_big_cells])
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+@pytest.mark.parametrize("internalized", [True, False])
+def test_cells_conflict_match_oracle(builder, reduce, internalized):
+    p, ag = builder()
+    q = _cells_problem(p)
+    f_or = sc.constant_field(q, [1, 1])
+    ag_or = orc.AgentSpec(ag.n, 2, 1)
+    for k in ("positions", "secretion", "saturation", "uptake", "net_export", "fraction_released",
+              "volumes"):
+        setattr(ag_or, k, getattr(ag, k).copy())
+    s = orc.Solver(q, f_or, agents=ag_or, compute_internalized=internalized)
+    d = gpu_domain(q, field=sc.constant_field(q, [1, 1]), agents=ag, internalized=internalized,
+                   reduce=reduce)
+    for _ in range(2):
+        s.solve(1)
+        d.step(1)
+        assert rel_linf(d.field(), f_or).max() < 1e-6
+        if internalized:
+            assert rel_linf(d.internalized(), ag_or.internalized).max() < 1e-9
+    for i in range(ag.n):
+        s.release(i)
+        d.release_internalized(i)
+    assert rel_linf(d.field(), f_or).max() < 1e-6
+
+
+def _random_agents(p, n, rng, clustered=False):
+    ag = orc.AgentSpec(n, p.S, p.dims)
+    box = np.array([p.nx * p.voxel[0], p.ny * p.voxel[1], p.nz * p.voxel[2]], dtype=np.float64)
+    if clustered:
+        pos = rng.normal(box / 2, box / 12, (n, 3))
+        pos = np.clip(pos, 0, np.nextafter(box, 0))
+    else:
+        pos = rng.uniform(0, box, (n, 3))
+    ag.positions[:] = pos[:, : p.dims]
+    ag.secretion[:] = rng.uniform(0, 10, (n, p.S))
+    ag.uptake[:] = rng.uniform(0, 10, (n, p.S))
+    ag.saturation[:] = rng.uniform(0, 2, (n, p.S))
+    ag.net_export[:] = rng.uniform(-1, 1, (n, p.S))
+    ag.fraction_released[:] = rng.uniform(0, 1, (n, p.S))
+    ag.volumes[:] = rng.uniform(1000, 3000, n)
+    ag.is_active[:] = rng.uniform(0, 1, n) < 0.95
+    return ag
+
+
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+@pytest.mark.parametrize("clustered", [False, True])
+def test_cell_sums_match_oracle(reduce, clustered):
+    rng = np.random.default_rng(11)
+    p = orc.Problem(3, (30, 20, 10), (10, 10, 10), 0.01, [0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
+    ag = _random_agents(p, 20000, rng, clustered)
+    s = orc.Solver(p, np.ones(p.shape), agents=ag)
+    s.solve(1)
+    num, den, fac, ballots, conflict = s.reduced()
+    d = gpu_domain(p, field=np.ones(p.shape), agents=ag, reduce=reduce)
+    owners, gn, gd, gf, gconf = d.cell_sums()
+    want_owners = np.unique(ballots[ballots >= 0])
+    if reduce == "sorted":
+        # same owners (lowest active index per voxel) and the same FP64 sums, bit for bit
+        assert np.array_equal(owners, want_owners)
+        assert np.array_equal(gn, num[owners]) and np.array_equal(gd, den[owners])
+        assert np.array_equal(gf, fac[owners])
+    else:
+        # atomic election may pick another owner; compare per voxel
+        vox_of = lambda o: (ag.positions[o] // 10).astype(int) @ np.array([1, 30, 600])
+        gmap = {int(v): k for k, v in enumerate(vox_of(owners))}
+        assert len(gmap) == len(want_owners)
+        for k, o in enumerate(want_owners):
+            j = gmap[int(vox_of(np.array([o]))[0])]
+            np.testing.assert_allclose(gn[j], num[o], rtol=1e-13, atol=1e-300)
+            np.testing.assert_allclose(gd[j], den[o], rtol=1e-13)
+            np.testing.assert_allclose(gf[j], fac[o], rtol=1e-12, atol=1e-18)
+    assert gconf == conflict
+
+
+def test_sorted_mode_is_reproducible():
+    rng = np.random.default_rng(12)
+    p, f0, dirs, bulk, _ = sc.config2(side=40, S=3, n_cells=0, seed=5)
+    ag = _random_agents(p, 30000, rng, clustered=True)
+    fields = []
+    for _ in range(2):
+        d = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True,
+                       reduce="sorted")
+        d.step(5)
+        fields.append((d.field(), d.internalized()))
+    assert np.array_equal(fields[0][0], fields[1][0])
+    assert np.array_equal(fields[0][1], fields[1][1])
+
+
+def test_cells_inactive_and_reactivation():
+    p, ag = sc.simple_cells(1)
+    q = _cells_problem(p)
+    ag.is_active[1] = 0
+    d = gpu_domain(q, field=sc.constant_field(q, [1, 1]), agents=ag, internalized=True)
+    d.step(1)
+    f = d.field(np.float64)
+    assert abs(f[0, 0, 0, 0] - 28) < 1e-5 and f[0, 0, 1, 0] == 1 and f[0, 0, 1, 1] == 1
+    assert np.all(d.internalized()[1] == 0)
+    # all inactive: nothing changes
+    ag.is_active[:] = 0
+    d2 = gpu_domain(q, field=sc.constant_field(q, [1, 1]), agents=ag, internalized=True)
+    d2.step(1)
+    assert np.all(d2.field() == 1) and np.all(d2.internalized() == 0)
+
+
+def test_agent_outside_grid_is_an_error():
+    p, ag = sc.simple_cells(3)
+    ag.positions[2] = [60.0, 10.0, 10.0]  # on the max face: index n
+    d = gpu_domain(_cells_problem(p), field=np.ones(p.shape), agents=ag)
+    with pytest.raises(Exception) as e:
+        d.step(1)
+    assert "outside" in str(e.value)
+
+
+# --------------------------------------------------------------- full steps
+
+def _full_run(p, f0, dirs, bulk, ag, steps, internalized, reduce="atomic", graph=0):
+    ag_or = None
+    if ag is not None:
+        ag_or = orc.AgentSpec(ag.n, p.S, p.dims)
+        for k in ("positions", "secretion", "saturation", "uptake", "net_export",
+                  "fraction_released", "volumes", "is_active"):
+            setattr(ag_or, k, getattr(ag, k).copy())
+    f_or = f0.astype(np.float32).astype(np.float64)
+    s = orc.Solver(p, f_or, dirichlet=dirs, bulk=bulk, agents=ag_or,
+                   compute_internalized=internalized)
+    s.solve(steps)
+    d = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized,
+                   reduce=reduce)
+    if graph:
+        d.step_graph(steps, graph)
+    else:
+        d.step(steps)
+    return d, f_or, ag_or
+
+
+def test_config1_full_100_steps():
+    p, f0, dirs, bulk, ag = sc.config1()
+    d, f_or, _ = _full_run(p, f0, dirs, bulk, ag, 100, internalized=False)
+    assert rel_linf(d.field(), f_or).max() < 1e-5
+
+
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+def test_config2_shape_reduced_grid(reduce):
+    """Config-2 physics (x-min Dirichlet, bulk sphere, uniform cells, 4 substrates
+    with D in [1, 1e5]) on a 40^3 grid for 60 steps, with internalized tracking."""
+    p, f0, dirs, bulk, ag = sc.config2(side=40, S=4, n_cells=4000, seed=7)
+    d, f_or, ag_or = _full_run(p, f0, dirs, bulk, ag, 60, internalized=True, reduce=reduce)
+    assert rel_linf(d.field(), f_or).max() < 1e-5
+    assert rel_linf(d.internalized(), ag_or.internalized).max() < 1e-5
+
+
+def test_graph_replay_matches_eager():
+    """A 10-step hipGraph replayed several times, with eager work and syncs in
+    between, equals the same steps launched eagerly."""
+    p, f0, dirs, bulk, ag = sc.config2(side=32, S=2, n_cells=2000, seed=9)
+    a = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True)
+    b = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True)
+    for _ in range(3):
+        a.step(10)
+        b.step_graph(10, 10)
+        b.sync()
+        assert np.array_equal(a.field(), b.field())
+        assert np.array_equal(a.internalized(), b.internalized())
+        a.total_mass()
+        b.total_mass()
+    a.step(7)
+    b.step_graph(7, 10)  # remainder steps run eagerly
+    assert np.array_equal(a.field(), b.field())
+
+
+def test_persistent_grid_same_result():
+    p, f0, dirs, bulk, ag = sc.config2(side=36, S=3, n_cells=1000, seed=10)
+    a = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag)
+    b = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag)
+    b.set_tuning(persistent_wgs_per_cu=2)
+    a.step(4)
+    b.step(4)
+    assert np.array_equal(a.field(), b.field())
+
+
+def test_config2_full_size_short():
+    """The 200^3 x 4 domain of the benchmark, 3 steps against the FP64 oracle."""
+    p, f0, dirs, bulk, ag = sc.config2(side=200, S=4, n_cells=100_000, seed=43)
+    d, f_or, _ = _full_run(p, f0, dirs, bulk, ag, 3, internalized=False)
+    assert rel_linf(d.field(), f_or).max() < 2e-6
