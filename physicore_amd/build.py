@@ -46,12 +46,47 @@ def build_hip():
     return out
 
 
+HOST_SRCS = ["mesh.cpp", "agent_data.cpp", "microenvironment.cpp", "microenvironment_builder.cpp",
+             "solver_registry.cpp", "hip_solver.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def build_host():
+    """C++20 hip_solver + the biofvm boundary it plugs into; talks to the GPU only
+    through libbfvm_hip.so's C ABI (no HIP headers)."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    out = os.path.join(LIB_DIR, "libbfvm_host.so")
+    srcs = [os.path.join(HOST, "src", f) for f in HOST_SRCS]
+    run([CXX, *HOST_FLAGS, "-shared", "-I", os.path.join(HOST, "include"), "-I", os.path.join(ROOT, "include"),
+         *srcs, "-L", LIB_DIR, "-lbfvm_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_cpp_tests():
+    """The reference's solver tests ported onto hip_solver (tests/cpp), linked to libbfvm_host.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_hip_solver.cpp")
+    if not os.path.exists(src):
+        return None
+    bindir = os.path.join(ROOT, "tests", "cpp", "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "test_hip_solver")
+    run([CXX, *HOST_FLAGS, "-I", os.path.join(HOST, "include"), "-I", os.path.join(ROOT, "include"),
+         src, "-L", LIB_DIR, "-lbfvm_host", "-lbfvm_hip", f"-Wl,-rpath,{LIB_DIR}",
+         "-Wl,-rpath,$ORIGIN/../../../physicore_amd/lib", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
 def build_all():
     build_hip()
+    build_host()
+    build_cpp_tests()
     build_oracle()
 
 
