@@ -176,11 +176,12 @@ int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 
 /* Launch parameters of the sweep kernels.  persistent_wgs_per_cu = 0 launches
  * one workgroup per tile; > 0 launches that many workgroups per CU, each
- * looping over tiles. */
+ * looping over tiles.  force_lds_sweeps = 1 solves every line inside an LDS
+ * tile instead of in registers (the variant used for lines longer than 256). */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
-	int32_t yz_columns; /* columns per y/z tile: 64 */
+	int32_t force_lds_sweeps;
 	int32_t reserved[6];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);

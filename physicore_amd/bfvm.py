@@ -41,7 +41,7 @@ class DomainDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("yz_columns", C.c_int32),
+    _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("force_lds_sweeps", C.c_int32),
                 ("reserved", C.c_int32 * 6)]
 
 
@@ -264,10 +264,11 @@ class Domain:
     def tuning(self):
         t = Tuning()
         self._check(lib().bfvm_get_tuning(self._h, C.byref(t)))
-        return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu, "yz_columns": t.yz_columns}
+        return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu,
+                "force_lds_sweeps": t.force_lds_sweeps}
 
-    def set_tuning(self, persistent_wgs_per_cu=0, yz_columns=64):
+    def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0):
         t = Tuning()
         t.persistent_wgs_per_cu = int(persistent_wgs_per_cu)
-        t.yz_columns = int(yz_columns)
+        t.force_lds_sweeps = int(force_lds_sweeps)
         self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))

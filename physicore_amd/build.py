@@ -37,11 +37,31 @@ def run(cmd, cwd=None):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
-def build_hip():
+HIP_SOURCES = ["bfvm_sweeps_y64.hip", "bfvm_sweeps_z64.hip", "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",
+               "bfvm_sweeps_rows.hip", "bfvm_sweeps.hip", "bfvm_kernels.hip", "bfvm_context.hip"]
+
+
+def build_hip(jobs=None):
+    """Compiles the HIP translation units in parallel (the unrolled sweep
+    instantiations dominate the build time) and links libbfvm_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIB_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, f) for f in ("bfvm_kernels.hip", "bfvm_context.hip")]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        run([HIPCC, *compile_flags, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    jobs = jobs or min(len(HIP_SOURCES), os.cpu_count() or 4, 16)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, HIP_SOURCES))
     out = os.path.join(LIB_DIR, "libbfvm_hip.so")
-    run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", out + ".tmp"])
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -166,6 +166,8 @@ struct bfvm_ctx
 	dvec<TileRef> tiles[3];
 	int ntiles[3] = { 0, 0, 0 };
 	int fac_cap[3] = { 0, 0, 0 };
+	int variant[3] = { 0, 0, 0 };
+	int nmax[3] = { 0, 0, 0 };
 	size_t lds[3] = { 0, 0, 0 };
 	bfvm_tuning tuning {};
 
@@ -278,30 +280,50 @@ int sync_domain(bfvm_ctx* c)
 	return BFVM_OK;
 }
 
+// Tiles of one axis for one domain (dom index `dom` of a launch).
+void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, std::vector<TileRef>& t)
+{
+	const int L = nx * S;
+	if (axis == 0)
+	{
+		const int R = x_rows_per_tile(S);
+		const int rows = ny * nz;
+		for (int r0 = 0; r0 < rows; r0 += R)
+			t.push_back({ dom, r0, std::min(R, rows - r0), 0 });
+		return;
+	}
+	const int outer = axis == 1 ? nz : ny;
+	if (variant == kSweepReg)
+	{
+		for (int f0 = 0; f0 < outer * L; f0 += kColsPerTile)
+			t.push_back({ dom, f0, 0, 0 });
+		return;
+	}
+	for (int o = 0; o < outer; o++)
+		for (int c0 = 0; c0 < L; c0 += kLinesPerTile)
+			t.push_back({ dom, o, c0, 0 });
+}
+
 int build_tiles(bfvm_ctx* c)
 {
 	for (int axis = 0; axis < 3; axis++)
 	{
 		std::vector<TileRef> t;
+		c->variant[axis] = kSweepLds;
+		c->nmax[axis] = 0;
 		if (axis < c->dims)
 		{
 			const int n = (int)c->n[axis];
-			if (axis == 0)
+			const int bucket = reg_bucket(n);
+			if (bucket && !c->tuning.force_lds_sweeps)
 			{
-				const int R = x_rows_per_tile(c->S);
-				const int rows = (int)(c->n[1] * c->n[2]);
-				for (int r0 = 0; r0 < rows; r0 += R)
-					t.push_back({ 0, r0, std::min(R, rows - r0), 0 });
+				c->variant[axis] = kSweepReg;
+				c->nmax[axis] = bucket;
 			}
-			else
-			{
-				const int outer = (int)(axis == 1 ? c->n[2] : c->n[1]);
-				for (int o = 0; o < outer; o++)
-					for (int c0 = 0; c0 < c->L; c0 += kLinesPerTile)
-						t.push_back({ 0, o, c0, 0 });
-			}
-			c->fac_cap[axis] = n * c->S;
-			c->lds[axis] = sweep_lds_bytes(axis, n, c->S, c->L, c->fac_cap[axis]);
+			domain_tiles(axis, c->variant[axis], 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, t);
+			// register kernels pad the factor rows to their bucket
+			c->fac_cap[axis] = (c->variant[axis] == kSweepReg ? c->nmax[axis] : n) * c->S;
+			c->lds[axis] = sweep_lds_bytes(axis, c->variant[axis], n, c->S, c->L, c->fac_cap[axis]);
 			if (c->lds[axis] > 160 * 1024)
 				return c->fail(BFVM_ERR_UNSUPPORTED, "grid axis " + std::to_string(axis) + " of length " +
 														 std::to_string(n) + " needs more than 160 KiB of LDS per tile");
@@ -329,8 +351,9 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 	for (int axis = 0; axis < 3; axis++)
 	{
 		if (axis < c->dims)
-			TRY_HIP(c, launch_sweep(axis, c->fp64, c->ddom.p, c->tiles[axis].p, c->ntiles[axis],
-									axis == c->dims - 1 ? 1 : 0, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
+			TRY_HIP(c, launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64,
+									axis == c->dims - 1 && c->face_mask != 0, c->ddom.p, c->tiles[axis].p,
+									c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
 									c->stream));
 		TRY_HIP(c, mark(axis + 1));
 	}
@@ -435,8 +458,8 @@ int valid_desc(const bfvm_domain_desc* d, std::string& why)
 	}
 	if ((double)d->grid[0] * d->grid[1] * d->grid[2] > 2147483647.0)
 		return why = "more than 2^31-1 voxels", 0;
-	if ((double)d->grid[0] * d->substrates > 2147483647.0)
-		return why = "row too long", 0;
+	if ((double)d->grid[0] * d->grid[1] * d->grid[2] * d->substrates * 4.0 > 2147483647.0)
+		return why = "field larger than 2 GiB (the sweeps address it with 32-bit byte offsets)", 0;
 	if (!(d->dt > 0) || !fp_ok(d->dt))
 		return why = "dt must be positive", 0;
 	if (!d->diffusion_coefficients || !d->decay_rates || !d->initial_conditions)
@@ -517,7 +540,7 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	c->lam.assign(desc->decay_rates, desc->decay_rates + c->S);
 	c->init.assign(desc->initial_conditions, desc->initial_conditions + c->S);
 	c->tuning.persistent_wgs_per_cu = 0;
-	c->tuning.yz_columns = kLinesPerTile;
+	c->tuning.force_lds_sweeps = 0;
 
 	auto bail = [&](int rc) {
 		g_error = c->err;
@@ -1027,9 +1050,13 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return BFVM_ERR_INVALID;
 	if (t->persistent_wgs_per_cu < 0 || t->persistent_wgs_per_cu > 64)
 		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [0, 64]");
-	if (t->yz_columns != kLinesPerTile)
-		return c->fail(BFVM_ERR_UNSUPPORTED, "yz_columns must be 64");
+	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
+		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->tuning = *t;
+	int rc = build_tiles(c);
+	if (rc)
+		return rc;
 	c->version++;
 	return BFVM_OK;
 }

@@ -70,9 +70,10 @@ struct DomainDev
 	int32_t pad1;
 };
 
-// One tile of a sweep: 64 lines of one domain.
-//   x: a = first row (row = y + ny*z), b = rows in the tile
-//   y: a = z, b = first column;  z: a = y, b = first column
+// One tile of a sweep.
+//   x (both variants): a = first row (row = y + ny*z), b = rows in the tile (64/S)
+//   y/z register variant: a = first flattened column (outer*L + x*S + s), 256 columns
+//   y/z LDS variant: a = outer (z for y, y for z), b = first column, 64 columns
 struct TileRef
 {
 	int32_t dom, a, b, pad;
@@ -84,19 +85,34 @@ struct SparseRef
 	int32_t dom, idx;
 };
 
-// LDS bytes a sweep tile needs for line length n and S substrates.
-// LDS bytes of the tile plus the factor tables for fac_cap doubles per table.
-size_t sweep_lds_bytes(int axis, int n, int S, int L, int fac_cap);
-int x_rows_per_tile(int S);
-int x_row_pitch(int L, int S);
+// Sweep kernel variants: lines held in VGPRs (n <= kRegMaxN, templated on a
+// bucket of 32) or solved inside an LDS tile (any n the LDS can hold).
+enum SweepVariant
+{
+	kSweepLds = 0,
+	kSweepReg = 1
+};
+constexpr int kRegMaxN = 256;
+constexpr int kColsPerTile = kSweepThreads; // y/z register tiles: 256 flattened columns
+int reg_bucket(int n);						// 32..256, or 0 when n > kRegMaxN
+
+// LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
+size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);
+// x tiles: 64/S rows; LDS row pitch Lp = L + pad with Lp = S (mod 64), so the
+// 64 lanes (r, s) reading element i of their lines hit distinct banks.
+__host__ __device__ inline int x_rows_per_tile(int S) { return S >= kLinesPerTile ? 1 : kLinesPerTile / S; }
+__host__ __device__ inline int x_row_pitch(int L, int S) { return L + ((S - L % 64) % 64 + 64) % 64; }
 
 // Raises the dynamic-LDS limit of the sweep kernels to 160 KiB (once per process).
 hipError_t init_sweep_kernels();
 
 // ---- launchers (enqueue on `stream`, return hipError_t) ----
-// fac_cap: doubles reserved per factor table in LDS (max n*S over the domains)
-hipError_t launch_sweep(int axis, bool fp64, const DomainDev* doms, const TileRef* tiles, int ntiles, int finish,
-						int fac_cap, size_t lds_bytes, int grid, hipStream_t stream);
+// fac_cap: doubles reserved per factor table in LDS (max n*S over the domains).
+// finish: apply face Dirichlet conditions in the last sweep of each domain.
+// faces: use the variant that applies face conditions (register y/z kernels).
+hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, const DomainDev* doms,
+						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
+						hipStream_t stream);
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
 hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
 hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* owner_refs, int64_t n, hipStream_t stream);

@@ -1,11 +1,11 @@
-// bfvm_kernels.hip -- CDNA4 (gfx950) kernels of the BioFVM backend.
-//
-// The step of the reference (reactions-diffusion/biofvm/kernels/openmp_solver/src/openmp_solver.cpp:23-40)
-// is diffusion (three Thomas sweeps, LOD) -> Dirichlet -> bulk -> cells.  Each
-// sweep is HBM-bound (about 0.6 flop per byte), so every kernel here is written
-// for bandwidth: a tile of 64 lines is staged through LDS with coalesced loads,
-// solved one line per lane, and written back with coalesced stores.  Face
-// Dirichlet conditions are applied while the last sweep writes its tile.
+// bfvm_kernels.hip -- the non-sweep kernels of the BioFVM backend on gfx950:
+// interior Dirichlet, bulk, the cell stage (recompute by CAS + wavefront-
+// aggregated atomics or by radix sort + segmented reduce, per-step apply,
+// internalized bookkeeping, release), field fill/convert, the FP64 mass
+// reduction, and the scan / radix sort they use.  The sweeps are in
+// bfvm_sweeps.hip.  Step order of the reference
+// (reactions-diffusion/biofvm/kernels/openmp_solver/src/openmp_solver.cpp:23-40):
+// diffusion -> Dirichlet -> bulk -> cells.
 #include "bfvm_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -15,187 +15,6 @@ namespace bfvm {
 namespace {
 
 constexpr int kWave = 64;
-
-__device__ __forceinline__ float apply_faces(const DomainDev& d, int x, int y, int z, int s, float v)
-{
-	// Later faces win: x-min, x-max, y-min, y-max, z-min, z-max
-	// (openmp_solver/src/dirichlet_solver.cpp:59-83), so test from the last.
-	const int m = d.face_mask;
-	const int S = d.S;
-	if ((m & 32) && z == d.nz - 1 && d.face_cond[5 * S + s])
-		return d.face_val[5 * S + s];
-	if ((m & 16) && z == 0 && d.face_cond[4 * S + s])
-		return d.face_val[4 * S + s];
-	if ((m & 8) && y == d.ny - 1 && d.face_cond[3 * S + s])
-		return d.face_val[3 * S + s];
-	if ((m & 4) && y == 0 && d.face_cond[2 * S + s])
-		return d.face_val[2 * S + s];
-	if ((m & 2) && x == d.nx - 1 && d.face_cond[1 * S + s])
-		return d.face_val[1 * S + s];
-	if ((m & 1) && x == 0 && d.face_cond[0 * S + s])
-		return d.face_val[0 * S + s];
-	return v;
-}
-
-// Thomas recurrence of one line held in LDS (openmp_solver/src/diffusion_solver.h:11-33):
-//   forward   d_i += e_{i-1} d_{i-1};  d_{n-1} *= b_{n-1}
-//   backward  d_i = (d_i + c d_{i+1}) b_i
-// ACC is the arithmetic type; the LDS copy of the line stays FP32 like the field.
-template <typename ACC>
-__device__ __forceinline__ void solve_line(float* line, int stride, int n, int S, int cs, const double* fb,
-										   const double* fe, double c)
-{
-	ACC prev = (ACC)line[0];
-	for (int i = 1; i < n; i++)
-	{
-		float* p = line + i * stride;
-		const ACC v = (ACC)(*p) + (ACC)fe[(i - 1) * S + cs] * prev;
-		*p = (float)v;
-		prev = v;
-	}
-	ACC nxt = prev * (ACC)fb[(n - 1) * S + cs];
-	line[(n - 1) * stride] = (float)nxt;
-	const ACC cc = (ACC)c;
-	for (int i = n - 2; i >= 0; i--)
-	{
-		float* p = line + i * stride;
-		nxt = ((ACC)(*p) + cc * nxt) * (ACC)fb[i * S + cs];
-		*p = (float)nxt;
-	}
-}
-
-// One kernel for the three axes.  LDS: [b: fac_cap][e: fac_cap][c: 64] doubles,
-// then the FP32 tile.  A workgroup loops over tiles (persistent when the grid
-// is smaller than the tile count) and reloads the factors only when the
-// domain changes.
-template <int AXIS, typename ACC>
-__global__ __launch_bounds__(kSweepThreads) void sweep_kernel(const DomainDev* __restrict__ doms,
-															  const TileRef* __restrict__ tiles, int ntiles,
-															  int finish, int fac_cap)
-{
-	extern __shared__ double lds[];
-	double* fb = lds;
-	double* fe = lds + fac_cap;
-	double* fc = lds + 2 * fac_cap;
-	float* tile = reinterpret_cast<float*>(lds + 2 * fac_cap + 64);
-
-	const int tid = threadIdx.x;
-	int cur = -1;
-
-	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
-	{
-		const TileRef tr = tiles[t];
-		const DomainDev& d = doms[tr.dom];
-		const int S = d.S;
-		const int n = AXIS == 0 ? d.nx : (AXIS == 1 ? d.ny : d.nz);
-
-		if (tr.dom != cur)
-		{
-			const double* gb = d.fb[AXIS];
-			const double* ge = d.fe[AXIS];
-			for (int k = tid; k < n * S; k += kSweepThreads)
-				fb[k] = gb[k];
-			for (int k = tid; k < (n - 1) * S; k += kSweepThreads)
-				fe[k] = ge[k];
-			for (int k = tid; k < S; k += kSweepThreads)
-				fc[k] = d.fc[AXIS][k];
-			cur = tr.dom;
-		}
-
-		const int L = d.L;
-		float* __restrict__ field = d.field;
-
-		if constexpr (AXIS == 0)
-		{
-			// tile = `rows` consecutive rows (contiguous in HBM), LDS row pitch Lp
-			// with Lp = S (mod 64) so the 64 lanes (r, s) hit distinct banks.
-			const int rows = tr.b;
-			const int Lp = L + ((S - L % 64) % 64 + 64) % 64;
-			const float* g = field + (size_t)tr.a * L;
-			const int total = rows * L;
-			{
-				int r = tid / L, j = tid - (tid / L) * L;
-				for (int idx = tid; idx < total; idx += kSweepThreads)
-				{
-					tile[r * Lp + j] = g[idx];
-					j += kSweepThreads;
-					while (j >= L)
-					{
-						j -= L;
-						r++;
-					}
-				}
-			}
-			__syncthreads();
-			if (tid < rows * S)
-			{
-				const int r = tid / S, s = tid - (tid / S) * S;
-				solve_line<ACC>(tile + r * Lp + s, S, n, S, s, fb, fe, fc[s]);
-			}
-			__syncthreads();
-			{
-				float* go = field + (size_t)tr.a * L;
-				int r = tid / L, j = tid - (tid / L) * L;
-				const bool faces = finish && d.face_mask;
-				for (int idx = tid; idx < total; idx += kSweepThreads)
-				{
-					float v = tile[r * Lp + j];
-					if (faces)
-					{
-						const int row = tr.a + r;
-						const int x = j / S, s = j - (j / S) * S;
-						v = apply_faces(d, x, row % d.ny, row / d.ny, s, v);
-					}
-					go[idx] = v;
-					j += kSweepThreads;
-					while (j >= L)
-					{
-						j -= L;
-						r++;
-					}
-				}
-			}
-		}
-		else
-		{
-			// tile = n rows x 64 consecutive columns (x*S + s) of one plane,
-			// line i of column j at LDS [i*64 + j].
-			const int c0 = tr.b;
-			const int w = min(kLinesPerTile, L - c0);
-			const size_t line_stride = AXIS == 1 ? (size_t)L : (size_t)d.ny * L;
-			const size_t base = AXIS == 1 ? (size_t)tr.a * d.ny * L : (size_t)tr.a * L;
-			const int j = tid & (kLinesPerTile - 1);
-			const int k0 = tid / kLinesPerTile;
-			const float* g = field + base + c0;
-			if (j < w)
-				for (int k = k0; k < n; k += kSweepThreads / kLinesPerTile)
-					tile[k * kLinesPerTile + j] = g[k * line_stride + j];
-			__syncthreads();
-			if (tid < w)
-			{
-				const int c = c0 + tid;
-				const int cs = d.coef_map ? d.coef_map[c] : c % S;
-				solve_line<ACC>(tile + tid, kLinesPerTile, n, S, cs, fb, fe, fc[cs]);
-			}
-			__syncthreads();
-			if (j < w)
-			{
-				float* go = field + base + c0;
-				const bool faces = finish && d.face_mask;
-				const int c = c0 + j;
-				const int x = c / S, s = c - (c / S) * S;
-				for (int k = k0; k < n; k += kSweepThreads / kLinesPerTile)
-				{
-					float v = tile[k * kLinesPerTile + j];
-					if (faces)
-						v = AXIS == 1 ? apply_faces(d, x, k, tr.a, s, v) : apply_faces(d, x, tr.a, k, s, v);
-					go[k * line_stride + j] = v;
-				}
-			}
-		}
-		__syncthreads();
-	}
-}
 
 // Interior Dirichlet voxels (dirichlet_solver.cpp:22-42).
 __global__ void dirichlet_interior_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
@@ -724,68 +543,6 @@ hipError_t launch1d(K kernel, int64_t n, int threads, hipStream_t stream, Args..
 } // namespace
 
 // ------------------------------------------------------------ launchers
-
-int x_rows_per_tile(int S) { return S >= kLinesPerTile ? 1 : kLinesPerTile / S; }
-
-int x_row_pitch(int L, int S) { return L + ((S - L % 64) % 64 + 64) % 64; }
-
-hipError_t init_sweep_kernels()
-{
-	const int max_lds = 160 * 1024;
-	const void* fns[] = { (const void*)sweep_kernel<0, double>, (const void*)sweep_kernel<1, double>,
-						  (const void*)sweep_kernel<2, double>, (const void*)sweep_kernel<0, float>,
-						  (const void*)sweep_kernel<1, float>,	(const void*)sweep_kernel<2, float> };
-	for (const void* f : fns)
-	{
-		hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-		if (e != hipSuccess)
-			return e;
-	}
-	return hipSuccess;
-}
-
-size_t sweep_lds_bytes(int axis, int n, int S, int L, int fac_cap)
-{
-	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
-	size_t tile;
-	if (axis == 0)
-		tile = (size_t)x_rows_per_tile(S) * x_row_pitch(L, S) * sizeof(float);
-	else
-		tile = (size_t)n * kLinesPerTile * sizeof(float);
-	return fac + tile;
-}
-
-hipError_t launch_sweep(int axis, bool fp64, const DomainDev* doms, const TileRef* tiles, int ntiles, int finish,
-							int fac_cap, size_t lds_bytes, int grid, hipStream_t stream)
-{
-	if (ntiles <= 0)
-		return hipSuccess;
-	if (grid <= 0 || grid > ntiles)
-		grid = ntiles;
-	const dim3 g(grid), b(kSweepThreads);
-#define BFVM_SWEEP(AX, T) \
-	hipLaunchKernelGGL((sweep_kernel<AX, T>), g, b, lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap)
-	if (fp64)
-	{
-		if (axis == 0)
-			BFVM_SWEEP(0, double);
-		else if (axis == 1)
-			BFVM_SWEEP(1, double);
-		else
-			BFVM_SWEEP(2, double);
-	}
-	else
-	{
-		if (axis == 0)
-			BFVM_SWEEP(0, float);
-		else if (axis == 1)
-			BFVM_SWEEP(1, float);
-		else
-			BFVM_SWEEP(2, float);
-	}
-#undef BFVM_SWEEP
-	return hipGetLastError();
-}
 
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
 {

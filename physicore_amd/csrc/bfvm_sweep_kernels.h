@@ -1,0 +1,506 @@
+// bfvm_sweep_kernels.h -- the three Thomas sweeps of the LOD step on gfx950
+// (device templates; instantiated by the bfvm_sweeps*.hip translation units).
+//
+// Reference: diffusion_solver::solve and solve_slice_{x,y,z}
+// (reactions-diffusion/biofvm/kernels/openmp_solver/src/diffusion_solver.cpp:116-202, 390-843),
+// recurrences of diffusion_solver.h:11-33:
+//   forward   d_i += e_{i-1} d_{i-1};  d_{n-1} *= b_{n-1}
+//   backward  d_i = (d_i + c d_{i+1}) b_i
+//
+// Every sweep reads and writes each FP32 value once, so all of them are bound
+// by HBM bandwidth.  Lines up to 256 long are held in VGPRs, one line per lane:
+//  * y and z: a workgroup owns 256 consecutive columns (x*S + s, flattened with
+//    the outer axis), so every load and store of a line step is one coalesced
+//    256-lane access; no LDS is needed for the data.
+//  * x: the lines of a row are interleaved by substrate, so a tile of 64/S rows
+//    is staged through LDS with 16-byte coalesced accesses and each lane then
+//    reads its strided line into registers (the transpose).
+// The Thomas factors (FP64) sit in LDS, transposed to [substrate][i] so each
+// lane reads them at an immediate offset from its own base.  Arithmetic is
+// FP64 (or FP32 on request); the field stays FP32.  Longer lines fall back to
+// a kernel that solves them inside an LDS tile.
+#pragma once
+
+#include "bfvm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
+
+namespace bfvm {
+namespace sweep {
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1.  Array
+// indices inside f are constants from the front end on, so a per-lane line
+// array is split into scalars (registers) before any other pass sees it.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+	(f(std::integral_constant<int, I> {}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+	static_for_impl(f, std::make_integer_sequence<int, N> {});
+}
+
+__device__ __forceinline__ float apply_faces(const DomainDev& d, int x, int y, int z, int s, float v)
+{
+	// Later faces win: x-min, x-max, y-min, y-max, z-min, z-max
+	// (openmp_solver/src/dirichlet_solver.cpp:59-83), so test from the last.
+	const int m = d.face_mask;
+	const int S = d.S;
+	if ((m & 32) && z == d.nz - 1 && d.face_cond[5 * S + s])
+		return d.face_val[5 * S + s];
+	if ((m & 16) && z == 0 && d.face_cond[4 * S + s])
+		return d.face_val[4 * S + s];
+	if ((m & 8) && y == d.ny - 1 && d.face_cond[3 * S + s])
+		return d.face_val[3 * S + s];
+	if ((m & 4) && y == 0 && d.face_cond[2 * S + s])
+		return d.face_val[2 * S + s];
+	if ((m & 2) && x == d.nx - 1 && d.face_cond[1 * S + s])
+		return d.face_val[1 * S + s];
+	if ((m & 1) && x == 0 && d.face_cond[0 * S + s])
+		return d.face_val[0 * S + s];
+	return v;
+}
+
+// Copies one axis' factors into LDS transposed to [s][i] with row stride npad:
+// fb[s*npad + i], fe[s*npad + i], fc[s].  Past the end of the line the tables
+// hold zeros (b_i = 0 for i >= n, e_i = 0 for i >= n-1), which decouples the
+// padding lanes of the register kernels: a padded element then contributes
+// exactly 0 to the last real one.
+__device__ __forceinline__ void load_factors(const DomainDev& d, int axis, int n, int npad, double* fb, double* fe,
+											 double* fc)
+{
+	const int S = d.S;
+	const double* gb = d.fb[axis];
+	const double* ge = d.fe[axis];
+	for (int k = threadIdx.x; k < npad * S; k += blockDim.x)
+	{
+		const int s = k / npad, i = k - (k / npad) * npad;
+		fb[k] = i < n ? gb[i * S + s] : 0.0;
+		fe[k] = i < n - 1 ? ge[i * S + s] : 0.0;
+	}
+	for (int k = threadIdx.x; k < S; k += blockDim.x)
+		fc[k] = d.fc[axis][k];
+}
+
+// Raw buffer access to the field: the resource holds the base address and the
+// size, a lane passes its own byte offset, the line step is a scalar offset.
+constexpr int kBufferWord3 = 0x00020000; // gfx9 raw buffer, 32-bit elements
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const DomainDev& d)
+{
+	const uint64_t bytes = (uint64_t)d.nx * d.ny * d.nz * d.S * sizeof(float);
+	const int num = (int)(uint32_t)(bytes > 0xffffffffull ? 0xffffffffull : bytes);
+	return __builtin_amdgcn_make_buffer_rsrc(d.field, (short)0, num, kBufferWord3);
+}
+
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int voff, int soff, float v)
+{
+	__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// Thomas solve of a line held in registers v[0..NMAX-1] (entries past the real
+// length are padding, decoupled by the zero factors).  No branches: the whole
+// solve is one basic block, so the loads feeding it can all be in flight.
+// `out(i, value)` receives the results from i = NMAX-1 down to 0; fb/fe point
+// at this line's substrate row.
+// The recurrences run in chunks of kChunk steps separated by scheduling
+// barriers, so the compiler neither hoists every factor read nor every store
+// offset to the top of the (fully unrolled) solve, which would exhaust VGPRs
+// and SGPRs; latency is hidden across the resident waves instead.
+constexpr int kChunk = 16;
+
+template <int NMAX, typename ACC, typename Out>
+__device__ __forceinline__ void thomas_regs(float (&v)[NMAX], const double* __restrict__ fb,
+											const double* __restrict__ fe, double c, Out out)
+{
+	constexpr int nchunks = (NMAX - 1 + kChunk - 1) / kChunk;
+	ACC prev = (ACC)v[0];
+	static_for<nchunks>([&](auto C) {
+		static_for<kChunk>([&](auto K) {
+			constexpr int i = 1 + C * kChunk + K;
+			if constexpr (i < NMAX)
+			{
+				prev = (ACC)v[i] + (ACC)fe[i - 1] * prev;
+				v[i] = (float)prev;
+			}
+		});
+		__builtin_amdgcn_sched_barrier(0);
+	});
+	const ACC cc = (ACC)c;
+	ACC nxt = (ACC)v[NMAX - 1] * (ACC)fb[NMAX - 1];
+	out(NMAX - 1, (float)nxt);
+	static_for<nchunks>([&](auto C) {
+		static_for<kChunk>([&](auto K) {
+			constexpr int i = NMAX - 2 - (C * kChunk + K);
+			if constexpr (i >= 0)
+			{
+				nxt = ((ACC)v[i] + cc * nxt) * (ACC)fb[i];
+				out(i, (float)nxt);
+			}
+		});
+		__builtin_amdgcn_sched_barrier(0);
+	});
+}
+
+// Face Dirichlet values seen by one line of the y or z sweep.  Faces of the
+// other axes that contain the line are constant along it; they split into
+// those ranked below the line's own two faces in the reference's order
+// (x-min < x-max < y-min < y-max < z-min < z-max, later wins;
+// openmp_solver/src/dirichlet_solver.cpp:59-83) and those ranked above.
+struct line_faces
+{
+	float lo_v, min_v, max_v, hi_v;
+	bool lo, has_min, has_max, hi;
+};
+
+__device__ __forceinline__ line_faces faces_of_line(const DomainDev& d, int axis, int x, int y, int z, int s)
+{
+	line_faces r {};
+	const int S = d.S, m = d.face_mask;
+	const int coord[3] = { x, y, z };
+	const int nn[3] = { d.nx, d.ny, d.nz };
+	for (int f = 0; f < kFaces; f++)
+	{
+		if (!((m >> f) & 1) || !d.face_cond[f * S + s])
+			continue;
+		const int a = f / 2;
+		const float v = d.face_val[f * S + s];
+		if (a == axis)
+		{
+			if (f & 1)
+			{
+				r.has_max = true;
+				r.max_v = v;
+			}
+			else
+			{
+				r.has_min = true;
+				r.min_v = v;
+			}
+			continue;
+		}
+		const bool on = (f & 1) ? coord[a] == nn[a] - 1 : coord[a] == 0;
+		if (!on)
+			continue;
+		if (a > axis)
+		{
+			r.hi = true;
+			r.hi_v = v;
+		}
+		else
+		{
+			r.lo = true;
+			r.lo_v = v;
+		}
+	}
+	return r;
+}
+
+// Value stored at position k of a line of length n: precedence applied in rank order.
+__device__ __forceinline__ float face_value(const line_faces& f, int k, int n, float v)
+{
+	float out = f.lo ? f.lo_v : v;
+	out = (k == 0 && f.has_min) ? f.min_v : out;
+	out = (k == n - 1 && f.has_max) ? f.max_v : out;
+	return f.hi ? f.hi_v : out;
+}
+
+// y (AXIS 1) and z (AXIS 2) sweeps, one line per lane in registers.  A tile is
+// 256 consecutive columns of the flattened (outer, x*S + s) space starting at
+// tr.a; outer is z for the y-sweep and y for the z-sweep.  For the z-sweep the
+// flattened columns are contiguous in memory; for the y-sweep they are
+// contiguous within each z plane.  Padding steps (i >= n) reload element n-1
+// and store to it before the real value, which is stored last.  FACES: the
+// launch finishes the step for domains whose last axis this is.
+template <int AXIS, int NMAX, typename ACC, bool FACES>
+__global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainDev* __restrict__ doms,
+																   const TileRef* __restrict__ tiles, int ntiles,
+																   int finish, int fac_cap)
+{
+	extern __shared__ double lds[];
+	double* fb = lds;
+	double* fe = lds + fac_cap;
+	double* fc = lds + 2 * fac_cap;
+	int cur = -1;
+
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+	{
+		const TileRef tr = tiles[t];
+		const DomainDev& d = doms[tr.dom];
+		const int n = AXIS == 1 ? d.ny : d.nz;
+		if (tr.dom != cur)
+		{
+			__syncthreads();
+			load_factors(d, AXIS, n, NMAX, fb, fe, fc);
+			__syncthreads();
+			cur = tr.dom;
+		}
+		const int S = d.S, L = d.L;
+		const int outer_n = AXIS == 1 ? d.nz : d.ny;
+		const int f = tr.a + (int)threadIdx.x;
+		if (f >= outer_n * L)
+			continue;
+		const int outer = f / L;
+		const int c = f - outer * L;
+		const int stride4 = (AXIS == 1 ? L : d.ny * L) * (int)sizeof(float);
+		const int voff = (AXIS == 1 ? outer * d.ny * L + c : f) * (int)sizeof(float);
+		const int cs = d.coef_map ? d.coef_map[c] : c % S;
+		const __amdgpu_buffer_rsrc_t rsrc = field_rsrc(d);
+
+		float v[NMAX];
+		static_for<NMAX>([&](auto I) { v[I] = buf_load(rsrc, voff, min((int)I, n - 1) * stride4); });
+		__builtin_amdgcn_sched_barrier(0);
+
+		// Opaque copies of the line geometry for the store phase: otherwise the
+		// per-step store offsets are recognised as the load offsets and all of
+		// them are kept live (in SGPRs) across the whole solve.
+		int sn = n - 1, sstride4 = stride4;
+		asm volatile("" : "+s"(sn), "+s"(sstride4));
+		if constexpr (FACES)
+		{
+			line_faces lf {};
+			if (finish && d.face_mask && AXIS == d.dims - 1)
+			{
+				const int x = c / S, s = c - (c / S) * S;
+				lf = AXIS == 1 ? faces_of_line(d, 1, x, 0, outer, s) : faces_of_line(d, 2, x, outer, 0, s);
+			}
+			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], [&](int i, float val) {
+				const int k = min(i, sn);
+				buf_store(rsrc, voff, k * sstride4, face_value(lf, k, sn + 1, val));
+			});
+		}
+		else
+		{
+			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs],
+								   [&](int i, float val) { buf_store(rsrc, voff, min(i, sn) * sstride4, val); });
+		}
+	}
+}
+
+// x sweep: a tile of `rows` = 64/S consecutive rows (contiguous in HBM) is
+// staged in LDS with row pitch Lp = S (mod 64), so the 64 lanes (r, s) read
+// their strided lines from distinct banks; each lane solves in registers and
+// writes back; the tile is stored coalesced.
+template <int NMAX, typename ACC>
+__global__ __launch_bounds__(kSweepThreads) void sweep_rows_kernel(const DomainDev* __restrict__ doms,
+																   const TileRef* __restrict__ tiles, int ntiles,
+																   int finish, int fac_cap)
+{
+	extern __shared__ double lds[];
+	double* fb = lds;
+	double* fe = lds + fac_cap;
+	double* fc = lds + 2 * fac_cap;
+	float* tile = reinterpret_cast<float*>(lds + 2 * fac_cap + 64);
+	const int tid = threadIdx.x;
+	int cur = -1;
+
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+	{
+		const TileRef tr = tiles[t];
+		const DomainDev& d = doms[tr.dom];
+		const int n = d.nx, S = d.S, L = d.L;
+		if (tr.dom != cur)
+		{
+			load_factors(d, 0, n, NMAX, fb, fe, fc); // the loop-end barrier of the previous tile orders this
+			cur = tr.dom;
+		}
+		const int rows = tr.b;
+		const int Lp = x_row_pitch(L, S);
+		const __amdgpu_buffer_rsrc_t rsrc = field_rsrc(d);
+		const int base4 = tr.a * L * (int)sizeof(float);
+		const int total = rows * L;
+		const bool vec = (S % 4 == 0) && (L % 4 == 0);
+		if (vec)
+		{
+			for (int q = tid; q < total / 4; q += kSweepThreads)
+			{
+				const int e = q * 4;
+				const int r = e / L, j = e - r * L;
+				*reinterpret_cast<u32x4*>(tile + r * Lp + j) = __builtin_amdgcn_raw_buffer_load_b128(rsrc, e * 4, base4, 0);
+			}
+		}
+		else
+		{
+			for (int e = tid; e < total; e += kSweepThreads)
+			{
+				const int r = e / L, j = e - r * L;
+				tile[r * Lp + j] = buf_load(rsrc, e * 4, base4);
+			}
+		}
+		__syncthreads();
+		if (tid < rows * S)
+		{
+			const int r = tid / S, s = tid - (tid / S) * S;
+			float* line = tile + r * Lp + s;
+			float v[NMAX];
+			static_for<NMAX>([&](auto I) { v[I] = line[min((int)I, n - 1) * S]; });
+			__builtin_amdgcn_sched_barrier(0);
+			int sn = n - 1, sS = S;
+			asm volatile("" : "+s"(sn), "+s"(sS));
+			thomas_regs<NMAX, ACC>(v, fb + s * NMAX, fe + s * NMAX, fc[s],
+								   [&](int i, float val) { line[min(i, sn) * sS] = val; });
+		}
+		__syncthreads();
+		const bool faces = finish && d.face_mask && d.dims == 1;
+		if (vec && !faces)
+		{
+			for (int q = tid; q < total / 4; q += kSweepThreads)
+			{
+				const int e = q * 4;
+				const int r = e / L, j = e - r * L;
+				__builtin_amdgcn_raw_buffer_store_b128(*reinterpret_cast<const u32x4*>(tile + r * Lp + j), rsrc, e * 4,
+													   base4, 0);
+			}
+		}
+		else
+		{
+			for (int e = tid; e < total; e += kSweepThreads)
+			{
+				const int r = e / L, j = e - r * L;
+				float v = tile[r * Lp + j];
+				if (faces)
+				{
+					const int row = tr.a + r;
+					const int x = j / S, s = j - (j / S) * S;
+					v = apply_faces(d, x, row % d.ny, row / d.ny, s, v);
+				}
+				buf_store(rsrc, e * 4, base4, v);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Lines longer than the register budget: the line stays in the LDS tile and
+// the recurrence walks it there (tile layouts as above; y/z tiles are 64
+// columns of one plane: tr.a = outer, tr.b = first column).
+template <int AXIS, typename ACC>
+__device__ __forceinline__ void thomas_lds(float* line, int stride, int n, const double* fb, const double* fe,
+										   double c)
+{
+	ACC prev = (ACC)line[0];
+	for (int i = 1; i < n; i++)
+	{
+		float* p = line + i * stride;
+		const ACC v = (ACC)(*p) + (ACC)fe[i - 1] * prev;
+		*p = (float)v;
+		prev = v;
+	}
+	ACC nxt = prev * (ACC)fb[n - 1];
+	line[(n - 1) * stride] = (float)nxt;
+	const ACC cc = (ACC)c;
+	for (int i = n - 2; i >= 0; i--)
+	{
+		float* p = line + i * stride;
+		nxt = ((ACC)(*p) + cc * nxt) * (ACC)fb[i];
+		*p = (float)nxt;
+	}
+}
+
+template <int AXIS, typename ACC>
+__global__ __launch_bounds__(kSweepThreads) void sweep_lds_kernel(const DomainDev* __restrict__ doms,
+																  const TileRef* __restrict__ tiles, int ntiles,
+																  int finish, int fac_cap)
+{
+	extern __shared__ double lds[];
+	double* fb = lds;
+	double* fe = lds + fac_cap;
+	double* fc = lds + 2 * fac_cap;
+	float* tile = reinterpret_cast<float*>(lds + 2 * fac_cap + 64);
+	const int tid = threadIdx.x;
+	int cur = -1;
+
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+	{
+		const TileRef tr = tiles[t];
+		const DomainDev& d = doms[tr.dom];
+		const int S = d.S, L = d.L;
+		const int n = AXIS == 0 ? d.nx : (AXIS == 1 ? d.ny : d.nz);
+		if (tr.dom != cur)
+		{
+			load_factors(d, AXIS, n, n, fb, fe, fc);
+			cur = tr.dom;
+		}
+		const bool faces = finish && d.face_mask && AXIS == d.dims - 1;
+		if constexpr (AXIS == 0)
+		{
+			const int rows = tr.b;
+			const int Lp = x_row_pitch(L, S);
+			float* g = d.field + (size_t)tr.a * L;
+			const int total = rows * L;
+			for (int e = tid; e < total; e += kSweepThreads)
+			{
+				const int r = e / L, j = e - r * L;
+				tile[r * Lp + j] = g[e];
+			}
+			__syncthreads();
+			if (tid < rows * S)
+			{
+				const int r = tid / S, s = tid - (tid / S) * S;
+				thomas_lds<AXIS, ACC>(tile + r * Lp + s, S, n, fb + s * n, fe + s * n, fc[s]);
+			}
+			__syncthreads();
+			for (int e = tid; e < total; e += kSweepThreads)
+			{
+				const int r = e / L, j = e - r * L;
+				float v = tile[r * Lp + j];
+				if (faces)
+				{
+					const int row = tr.a + r;
+					v = apply_faces(d, j / S, row % d.ny, row / d.ny, j % S, v);
+				}
+				g[e] = v;
+			}
+		}
+		else
+		{
+			const int c0 = tr.b;
+			const int w = min(kLinesPerTile, L - c0);
+			const size_t line_stride = AXIS == 1 ? (size_t)L : (size_t)d.ny * L;
+			const size_t base = AXIS == 1 ? (size_t)tr.a * d.ny * L : (size_t)tr.a * L;
+			const int j = tid & (kLinesPerTile - 1);
+			const int k0 = tid / kLinesPerTile;
+			float* g = d.field + base + c0;
+			if (j < w)
+				for (int k = k0; k < n; k += kSweepThreads / kLinesPerTile)
+					tile[k * kLinesPerTile + j] = g[k * line_stride + j];
+			__syncthreads();
+			if (tid < w)
+			{
+				const int c = c0 + tid;
+				const int cs = d.coef_map ? d.coef_map[c] : c % S;
+				thomas_lds<AXIS, ACC>(tile + tid, kLinesPerTile, n, fb + cs * n, fe + cs * n, fc[cs]);
+			}
+			__syncthreads();
+			if (j < w)
+			{
+				const int c = c0 + j;
+				const int x = c / S, s = c - (c / S) * S;
+				for (int k = k0; k < n; k += kSweepThreads / kLinesPerTile)
+				{
+					float v = tile[k * kLinesPerTile + j];
+					if (faces)
+						v = AXIS == 1 ? apply_faces(d, x, k, tr.a, s, v) : apply_faces(d, x, tr.a, k, s, v);
+					g[k * line_stride + j] = v;
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+using kernel_fn = void (*)(const DomainDev*, const TileRef*, int, int, int);
+
+} // namespace sweep
+} // namespace bfvm

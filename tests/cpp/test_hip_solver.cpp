@@ -583,8 +583,9 @@ void simple_case(index_t dims, bool internalized, bool recompute)
 	m->solver->transfer_to_host(*m);
 	if (internalized)
 	{
-		EXPECT_EQ(at(0, 0), 1.0);
-		EXPECT_EQ(at(0, 1), 1.0);
+		// exactly 1 in the reference's FP64; within FP32 rounding here
+		EXPECT_NEAR(at(0, 0), 1.0, 1e-6);
+		EXPECT_NEAR(at(0, 1), 1.0, 1e-6);
 		EXPECT_EQ(a1->internalized_substrates()[0], 0.0);
 		EXPECT_EQ(a1->internalized_substrates()[1], 0.0);
 	}
