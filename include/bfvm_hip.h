@@ -177,12 +177,15 @@ int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 /* Launch parameters of the sweep kernels.  persistent_wgs_per_cu = 0 launches
  * one workgroup per tile; > 0 launches that many workgroups per CU, each
  * looping over tiles.  force_lds_sweeps = 1 solves every line inside an LDS
- * tile instead of in registers (the variant used for lines longer than 256). */
+ * tile instead of in registers (the variant used for lines longer than 256).
+ * no_xscan = 1 solves x lines through an LDS transpose instead of by the
+ * lane-parallel affine scan. */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
 	int32_t force_lds_sweeps;
-	int32_t reserved[6];
+	int32_t no_xscan;
+	int32_t reserved[5];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
 int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);

@@ -38,7 +38,8 @@ def run(cmd, cwd=None):
 
 
 HIP_SOURCES = ["bfvm_sweeps_y64.hip", "bfvm_sweeps_z64.hip", "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",
-               "bfvm_sweeps_rows.hip", "bfvm_sweeps.hip", "bfvm_kernels.hip", "bfvm_context.hip"]
+               "bfvm_sweeps_rows.hip", "bfvm_sweeps_xscan.hip", "bfvm_sweeps.hip", "bfvm_kernels.hip",
+               "bfvm_context.hip"]
 
 
 def build_hip(jobs=None):

@@ -155,7 +155,8 @@ struct bfvm_ctx
 	dvec<uint8_t> is_active;
 	std::vector<double> h_positions;
 	dvec<int32_t> cell_voxel, owner_of, ballot, owners, flags;
-	dvec<double> num, den, fac, rnum, rden, rfac, dnew;
+	dvec<double> num, den, fac, rnum, rden, rfac, dnew, pnum, pden, pfac;
+	dvec<int32_t> pvox, slot_of;
 	dvec<SparseRef> agent_refs; // (0, k) for k < n_agents
 	dvec<uint32_t> keys, vals, tkeys, tvals;
 	dvec<int32_t> hist, scan_tmp, heads, head_scan;
@@ -273,6 +274,11 @@ int sync_domain(bfvm_ctx* c)
 	d.rfac = c->rfac.p;
 	d.dnew = c->dnew.p;
 	d.owners = c->owners.p;
+	d.pvox = c->pvox.p;
+	d.slot_of = c->slot_of.p;
+	d.pnum = c->pnum.p;
+	d.pden = c->pden.p;
+	d.pfac = c->pfac.p;
 	d.flags = c->flags.p;
 	d.compute_internalized = c->internalized ? 1 : 0;
 	TRY_HIP(c, c->ddom.upload(&d, 1, c->stream));
@@ -284,6 +290,13 @@ int sync_domain(bfvm_ctx* c)
 void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, std::vector<TileRef>& t)
 {
 	const int L = nx * S;
+	if (axis == 0 && variant == kSweepScan)
+	{
+		const int rows = ny * nz;
+		for (int r0 = 0; r0 < rows; r0 += kScanRowsPerTile)
+			t.push_back({ dom, r0, std::min(kScanRowsPerTile, rows - r0), 0 });
+		return;
+	}
 	if (axis == 0)
 	{
 		const int R = x_rows_per_tile(S);
@@ -315,7 +328,15 @@ int build_tiles(bfvm_ctx* c)
 		{
 			const int n = (int)c->n[axis];
 			const int bucket = reg_bucket(n);
-			if (bucket && !c->tuning.force_lds_sweeps)
+			const int vpl = xscan_vpl(n, c->S);
+			// x: the scan kernel unless faces must be applied in this sweep (1-D domains)
+			if (axis == 0 && vpl && !(c->dims == 1 && c->face_mask) && !c->tuning.force_lds_sweeps &&
+				!c->tuning.no_xscan)
+			{
+				c->variant[axis] = kSweepScan;
+				c->nmax[axis] = vpl;
+			}
+			else if (bucket && !c->tuning.force_lds_sweeps)
 			{
 				c->variant[axis] = kSweepReg;
 				c->nmax[axis] = bucket;
@@ -352,7 +373,7 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 	{
 		if (axis < c->dims)
 			TRY_HIP(c, launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64,
-									axis == c->dims - 1 && c->face_mask != 0, c->ddom.p, c->tiles[axis].p,
+									axis == c->dims - 1 && c->face_mask != 0, c->S, c->ddom.p, c->tiles[axis].p,
 									c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
 									c->stream));
 		TRY_HIP(c, mark(axis + 1));
@@ -361,13 +382,13 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 		TRY_HIP(c, launch_dirichlet_interior(c->ddom.p, c->dir_refs.p, c->dir_count, c->stream));
 	TRY_HIP(c, mark(4));
 	if (c->bulk_count)
-		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_refs.p, c->bulk_count, c->stream));
+		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_refs.p, c->bulk_count, c->S, c->stream));
 	TRY_HIP(c, mark(5));
 	if (c->n_owners > 0)
 	{
-		TRY_HIP(c, launch_cells_apply(c->ddom.p, c->agent_refs.p, c->n_owners, c->stream));
+		TRY_HIP(c, launch_cells_apply(c->ddom.p, c->agent_refs.p, c->n_owners, c->S, c->stream));
 		if (c->internalized && c->conflict)
-			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->agent_refs.p, c->n_agents, c->stream));
+			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->agent_refs.p, c->n_agents, c->S, c->stream));
 	}
 	TRY_HIP(c, mark(6));
 	return BFVM_OK;
@@ -436,6 +457,11 @@ int recompute(bfvm_ctx* c)
 	}
 	c->conflict = f[0] != 0;
 	c->n_owners = f[2];
+	if (c->n_owners > 0)
+	{
+		TRY_HIP(c, launch_cells_pack(c->ddom.p, c->n_owners, s));
+		TRY_HIP(c, hipStreamSynchronize(s));
+	}
 	return BFVM_OK;
 }
 
@@ -541,6 +567,7 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	c->init.assign(desc->initial_conditions, desc->initial_conditions + c->S);
 	c->tuning.persistent_wgs_per_cu = 0;
 	c->tuning.force_lds_sweeps = 0;
+	c->tuning.no_xscan = 0;
 
 	auto bail = [&](int rc) {
 		g_error = c->err;
@@ -633,7 +660,14 @@ int bfvm_set_dirichlet_faces(bfvm_ctx* c, const double* values, const uint8_t* c
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	TRY_HIP(c, c->face_val.upload(v.data(), v.size(), c->stream));
 	TRY_HIP(c, c->face_cond.upload(k.data(), k.size(), c->stream));
+	const bool rebuild = c->dims == 1 && ((c->face_mask != 0) != (mask != 0));
 	c->face_mask = mask;
+	if (rebuild)
+	{
+		int rc = build_tiles(c);
+		if (rc)
+			return rc;
+	}
 	return sync_domain(c);
 }
 
@@ -762,6 +796,11 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->rden.alloc(nS));
 	TRY_HIP(c, c->rfac.alloc(nS));
 	TRY_HIP(c, c->dnew.alloc(nS));
+	TRY_HIP(c, c->pnum.alloc(nS));
+	TRY_HIP(c, c->pden.alloc(nS));
+	TRY_HIP(c, c->pfac.alloc(nS));
+	TRY_HIP(c, c->pvox.alloc(n));
+	TRY_HIP(c, c->slot_of.alloc(n));
 	auto r = refs_for(n);
 	TRY_HIP(c, c->agent_refs.upload(r.data(), r.size(), c->stream));
 	c->recompute_pending = true;
@@ -1052,6 +1091,8 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [0, 64]");
 	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
+	if (t->no_xscan != 0 && t->no_xscan != 1)
+		return c->fail(BFVM_ERR_INVALID, "no_xscan must be 0 or 1");
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->tuning = *t;
 	int rc = build_tiles(c);

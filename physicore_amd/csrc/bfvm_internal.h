@@ -63,8 +63,15 @@ struct DomainDev
 	double* rnum; // [n*S] reduced sums, indexed by owner
 	double* rden;
 	double* rfac;
-	double* dnew; // [n*S] new density at the owner's voxel (FP64)
-	int32_t* owners; // [n] owner agents
+	int32_t* owners; // [n] owner agents (the first n_owners entries)
+	// apply list, packed at recompute in owner-list order k: voxel, sums, and
+	// the new density at that voxel (FP64) for the per-agent update
+	int32_t* pvox;	  // [n_owners]
+	int32_t* slot_of; // [n] list slot of an owner agent
+	double* pnum;	  // [n_owners*S]
+	double* pden;
+	double* pfac;
+	double* dnew;
 	int32_t* flags;	 // [0] conflict, [1] agent outside grid, [2] owner count
 	int32_t compute_internalized;
 	int32_t pad1;
@@ -90,8 +97,20 @@ struct SparseRef
 enum SweepVariant
 {
 	kSweepLds = 0,
-	kSweepReg = 1
+	kSweepReg = 1,
+	kSweepScan = 2 // x only: lanes along the row, affine scans; nmax carries voxels per lane
 };
+constexpr int kScanRowsPerTile = kSweepThreads / 64; // one row per wave
+// voxels per lane of the x-scan kernel for a row of nx voxels and S substrates, 0 if unsupported
+__host__ inline int xscan_vpl(int nx, int S)
+{
+	if (S < 1 || S > 8)
+		return 0;
+	for (int v = 1; v <= 8; v *= 2)
+		if (64 * v >= nx && S * v <= 32)
+			return v;
+	return 0;
+}
 constexpr int kRegMaxN = 256;
 constexpr int kColsPerTile = kSweepThreads; // y/z register tiles: 256 flattened columns
 int reg_bucket(int n);						// 32..256, or 0 when n > kRegMaxN
@@ -110,13 +129,17 @@ hipError_t init_sweep_kernels();
 // fac_cap: doubles reserved per factor table in LDS (max n*S over the domains).
 // finish: apply face Dirichlet conditions in the last sweep of each domain.
 // faces: use the variant that applies face conditions (register y/z kernels).
-hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, const DomainDev* doms,
+// S: substrate count (selects the x-scan instantiation; all its domains share it).
+hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, int S, const DomainDev* doms,
 						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
 						hipStream_t stream);
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
-hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
-hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* owner_refs, int64_t n, hipStream_t stream);
-hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* cell_refs, int64_t n,
+// The per-step sparse stages run one thread per (entry, substrate); `S` is the
+// substrate count of every referenced domain (batched launches group by S).
+hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, int S, hipStream_t stream);
+hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* owner_refs, int64_t n, int S,
+							  hipStream_t stream);
+hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* cell_refs, int64_t n, int S,
 									 hipStream_t stream);
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
 hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);
@@ -127,6 +150,8 @@ hipError_t launch_release(const DomainDev* dom, int64_t agent, int64_t voxel, hi
 hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream);
 hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stream);
 hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream);
+// packs the apply list from owners[0..n_owners) (both modes)
+hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream);
 // sorted mode: keys/vals are scratch of n entries (sorted in place by radix_sort_pairs)
 hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
 								  hipStream_t stream);

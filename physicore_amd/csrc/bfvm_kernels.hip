@@ -33,20 +33,20 @@ __global__ void dirichlet_interior_kernel(const DomainDev* __restrict__ doms, co
 }
 
 // Bulk supply/uptake (bulk_solver.cpp:12-41): D = (D + dt*S*T) / (1 + dt*(U+S)),
-// with a = dt*S*T and b = 1 + dt*(U+S) precomputed in FP64 on the host.
-__global__ void bulk_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs, int64_t n)
+// with a = dt*S*T and b = 1 + dt*(U+S) precomputed in FP64 on the host.  One
+// thread per (listed voxel, substrate).
+__global__ void bulk_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs, int64_t n, int S)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n)
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * S)
 		return;
+	const int64_t k = t / S;
+	const int s = (int)(t - k * S);
 	const SparseRef r = refs[k];
 	const DomainDev& d = doms[r.dom];
-	const int S = d.S;
-	float* D = d.field + d.bulk_vox[r.idx] * S;
-	const double* a = d.bulk_a + (int64_t)r.idx * S;
-	const double* b = d.bulk_b + (int64_t)r.idx * S;
-	for (int s = 0; s < S; s++)
-		D[s] = (float)(((double)D[s] + a[s]) / b[s]);
+	const int64_t q = (int64_t)r.idx * S + s;
+	float* D = d.field + d.bulk_vox[r.idx] * S + s;
+	*D = (float)(((double)*D + d.bulk_a[q]) / d.bulk_b[q]);
 }
 
 // ---------------------------------------------------------------- cells
@@ -245,58 +245,72 @@ __global__ void cells_segment_reduce_kernel(const DomainDev* __restrict__ dp, co
 	}
 }
 
+// Packs the apply list in owner-list order: the owner's voxel and its FP64
+// sums, so the per-step kernels read them coalesced.
+__global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_owners)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n_owners)
+		return;
+	const DomainDev& d = *dp;
+	const int o = d.owners[k];
+	const int S = d.S;
+	d.pvox[k] = d.cell_voxel[o];
+	d.slot_of[o] = (int32_t)k;
+	for (int s = 0; s < S; s++)
+	{
+		d.pnum[k * S + s] = d.rnum[(int64_t)o * S + s];
+		d.pden[k * S + s] = d.rden[(int64_t)o * S + s];
+		d.pfac[k * S + s] = d.rfac[(int64_t)o * S + s];
+	}
+}
+
 // compute_result, owner part (cell_solver.cpp:187-227, compute_densities :146-163,
 // compute_fused :165-185): D = (D + sum num) / sum den + sum fac at each occupied
 // voxel.  Without a conflict and with tracking on, the owner's internalized
 // amount takes V*(D_old - D_new); otherwise D_new is kept in FP64 for the
-// per-agent update.
-__global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
-								   int64_t n)
+// per-agent update.  One thread per (occupied voxel, substrate).  Agent
+// activity is fixed between recomputes (new agent arrays always trigger one),
+// so every listed owner is active.
+__global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs, int64_t n,
+								   int S)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n)
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * S)
 		return;
-	const SparseRef r = refs[k];
+	const int64_t kk = t / S;
+	const int s = (int)(t - kk * S);
+	const SparseRef r = refs[kk];
 	const DomainDev& d = doms[r.dom];
-	const int o = d.owners[r.idx];
-	if (!d.is_active[o])
-		return;
-	const int S = d.S;
-	float* D = d.field + (int64_t)d.cell_voxel[o] * S;
-	const bool fused = d.compute_internalized && !d.flags[0];
-	for (int s = 0; s < S; s++)
-	{
-		const int64_t q = (int64_t)o * S + s;
-		const double d0 = (double)D[s];
-		const double d1 = (d0 + d.rnum[q]) / d.rden[q] + d.rfac[q];
-		D[s] = (float)d1;
-		if (fused)
-			d.internalized[q] += d.voxel_volume * (d0 - d1);
-		else
-			d.dnew[q] = d1;
-	}
+	const int64_t q = (int64_t)r.idx * S + s;
+	float* D = d.field + (int64_t)d.pvox[r.idx] * S + s;
+	const double d0 = (double)*D;
+	const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
+	*D = (float)d1;
+	if (d.compute_internalized && !d.flags[0])
+		d.internalized[(int64_t)d.owners[r.idx] * S + s] += d.voxel_volume * (d0 - d1);
+	else
+		d.dnew[q] = d1;
 }
 
 // compute_internalized (cell_solver.cpp:131-144, :229-243):
 // I -= V*(num - D_new*den + fac) for every active agent, when voxels are shared.
 __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
-										  int64_t n)
+										  int64_t n, int S)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n)
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * S)
 		return;
-	const SparseRef r = refs[k];
+	const int64_t kk = t / S;
+	const int s = (int)(t - kk * S);
+	const SparseRef r = refs[kk];
 	const DomainDev& d = doms[r.dom];
 	const int64_t i = r.idx;
-	if (!d.is_active[i] || d.cell_voxel[i] < 0)
+	if (d.cell_voxel[i] < 0)
 		return;
-	const int S = d.S;
-	const int64_t o = d.owner_of[i];
-	for (int s = 0; s < S; s++)
-	{
-		const int64_t q = i * S + s;
-		d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[o * S + s] * d.den[q] + d.fac[q]);
-	}
+	const int64_t q = i * S + s;
+	const int64_t slot = d.slot_of[d.owner_of[i]];
+	d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[slot * S + s] * d.den[q] + d.fac[q]);
 }
 
 // release_internal (cell_solver.cpp:274-289): D += I*F/V, I = 0.
@@ -549,19 +563,25 @@ hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* ref
 	return launch1d(dirichlet_interior_kernel, n, 256, stream, doms, refs, n);
 }
 
-hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, int S, hipStream_t stream)
 {
-	return launch1d(bulk_kernel, n, 256, stream, doms, refs, n);
+	return launch1d(bulk_kernel, n * S, 256, stream, doms, refs, n, S);
 }
 
-hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* refs, int64_t n, int S, hipStream_t stream)
 {
-	return launch1d(cells_apply_kernel, n, 256, stream, doms, refs, n);
+	return launch1d(cells_apply_kernel, n * S, 256, stream, doms, refs, n, S);
 }
 
-hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream)
+hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* refs, int64_t n, int S,
+									 hipStream_t stream)
 {
-	return launch1d(cells_internalized_kernel, n, 256, stream, doms, refs, n);
+	return launch1d(cells_internalized_kernel, n * S, 256, stream, doms, refs, n, S);
+}
+
+hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream)
+{
+	return launch1d(cells_pack_kernel, n_owners, 256, stream, dom, n_owners);
 }
 
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream)

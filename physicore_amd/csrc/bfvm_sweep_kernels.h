@@ -504,3 +504,244 @@ using kernel_fn = void (*)(const DomainDev*, const TileRef*, int, int, int);
 
 } // namespace sweep
 } // namespace bfvm
+
+namespace bfvm {
+namespace sweep {
+
+// ------------------------------------------------------------------------
+// x sweep by affine scans (lanes along the row).
+//
+// Both Thomas recurrences are first-order linear recurrences,
+//   forward   y_x = e_{x-1} y_{x-1} + d_x
+//   backward  z_x = (c b_x) z_{x+1} + b_x y_x
+// so a lane holding VPL consecutive voxels (all S substrates, contiguous in
+// HBM) turns its chunk into one affine map per substrate, Y_out = A*Y_in + B.
+// The maps are combined across the 64 lanes of the wave with a log-depth scan
+// (DPP shifts inside rows of 16 lanes, lane reads across rows), after which
+// each lane re-runs its chunk from the correct carry.  One wave solves one row
+// (all S lines of it); loads and stores are fully coalesced, nothing goes
+// through LDS, and the lane's factors stay in registers across rows.
+// Padding (x >= nx) has e = b = 0, which makes it inert in both directions.
+
+template <typename T>
+__device__ __forceinline__ T dpp_identity_a()
+{
+	return (T)1;
+}
+
+template <int CTRL, int ROW_MASK, typename T>
+__device__ __forceinline__ T dpp_mov(T old, T src)
+{
+	if constexpr (sizeof(T) == 8)
+	{
+		const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+		const unsigned long long v = __builtin_bit_cast(unsigned long long, src);
+		const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)v, CTRL,
+																   ROW_MASK, 0xf, false);
+		const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(v >> 32),
+																   CTRL, ROW_MASK, 0xf, false);
+		return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+	}
+	else
+	{
+		return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+																  __builtin_bit_cast(int, src), CTRL, ROW_MASK, 0xf,
+																  false));
+	}
+}
+
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int lane)
+{
+	if constexpr (sizeof(T) == 8)
+	{
+		const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+		const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+		const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+		return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+	}
+	else
+	{
+		return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+	}
+}
+
+// (A, B) <- (A, B) o (A', B'): apply the other map first, then this one.
+template <typename T>
+__device__ __forceinline__ void compose(T& A, T& B, T Ao, T Bo)
+{
+	B = A * Bo + B;
+	A = A * Ao;
+}
+
+// Inclusive scan of affine maps towards higher lanes (lane l ends with the
+// composition of lanes 0..l).  DPP row_shr inside rows of 16, then
+// row_bcast:15 / row_bcast:31 across rows (GFX9 DPP).
+template <typename T>
+__device__ __forceinline__ void scan_up(T& A, T& B)
+{
+	const T one = (T)1, zero = (T)0;
+	compose(A, B, dpp_mov<0x111, 0xf>(one, A), dpp_mov<0x111, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x112, 0xf>(one, A), dpp_mov<0x112, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x114, 0xf>(one, A), dpp_mov<0x114, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x118, 0xf>(one, A), dpp_mov<0x118, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x142, 0xa>(one, A), dpp_mov<0x142, 0xa>(zero, B));
+	compose(A, B, dpp_mov<0x143, 0xc>(one, A), dpp_mov<0x143, 0xc>(zero, B));
+}
+
+// Inclusive scan towards lower lanes (lane l ends with lanes l..63, the higher
+// lane applied first): row_shl inside rows, then each row composes with the
+// totals of the rows above it, read from their first lanes.
+template <typename T>
+__device__ __forceinline__ void scan_down(T& A, T& B, int lane)
+{
+	const T one = (T)1, zero = (T)0;
+	compose(A, B, dpp_mov<0x101, 0xf>(one, A), dpp_mov<0x101, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x102, 0xf>(one, A), dpp_mov<0x102, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x104, 0xf>(one, A), dpp_mov<0x104, 0xf>(zero, B));
+	compose(A, B, dpp_mov<0x108, 0xf>(one, A), dpp_mov<0x108, 0xf>(zero, B));
+	T A3 = read_lane(A, 48), B3 = read_lane(B, 48);
+	T A2 = read_lane(A, 32), B2 = read_lane(B, 32);
+	T A1 = read_lane(A, 16), B1 = read_lane(B, 16);
+	// totals of rows 3; 2..3; 1..3
+	compose(A2, B2, A3, B3);
+	compose(A1, B1, A2, B2);
+	const int row = lane >> 4;
+	const T Ar = row == 0 ? A1 : (row == 1 ? A2 : (row == 2 ? A3 : one));
+	const T Br = row == 0 ? B1 : (row == 1 ? B2 : (row == 2 ? B3 : zero));
+	compose(A, B, Ar, Br);
+}
+
+template <int S, int VPL, typename ACC>
+__global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const DomainDev* __restrict__ doms,
+																	const TileRef* __restrict__ tiles, int ntiles,
+																	int finish, int fac_cap)
+{
+	(void)finish;
+	(void)fac_cap;
+	constexpr int C = S * VPL; // floats per lane
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	int cur = -1;
+	int nx = 0;
+	ACC fa[C], fbv[C], cs[S];
+
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+	{
+		const TileRef tr = tiles[t];
+		const DomainDev& d = doms[tr.dom];
+		if (tr.dom != cur)
+		{
+			nx = d.nx;
+			const double* ge = d.fe[0];
+			const double* gb = d.fb[0];
+			static_for<C>([&](auto K) {
+				constexpr int j = K / S, s = K % S;
+				const int x = lane * VPL + j;
+				fa[K] = (x >= 1 && x < nx) ? (ACC)ge[(x - 1) * S + s] : (ACC)0;
+				fbv[K] = x < nx ? (ACC)gb[x * S + s] : (ACC)0;
+			});
+			static_for<S>([&](auto s) { cs[s] = (ACC)d.fc[0][s]; });
+			cur = tr.dom;
+		}
+		if (wave >= tr.b)
+			continue;
+		const int row = tr.a + wave;
+		const int L = nx * S;
+		const __amdgpu_buffer_rsrc_t rsrc = field_rsrc(d);
+		const int rowoff = row * L * (int)sizeof(float);
+		const int loff = lane * C * (int)sizeof(float);
+		const bool full = lane * VPL + VPL <= nx;
+		const bool vec = (C % 4 == 0) && (L % 4 == 0);
+
+		float v[C];
+		if (vec && full)
+		{
+			static_for<C / 4>([&](auto Q) {
+				const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rsrc, loff + Q * 16, rowoff, 0);
+				v[4 * Q + 0] = __builtin_bit_cast(float, w.x);
+				v[4 * Q + 1] = __builtin_bit_cast(float, w.y);
+				v[4 * Q + 2] = __builtin_bit_cast(float, w.z);
+				v[4 * Q + 3] = __builtin_bit_cast(float, w.w);
+			});
+		}
+		else
+		{
+			static_for<C>([&](auto K) {
+				constexpr int j = K / S;
+				v[K] = lane * VPL + j < nx ? buf_load(rsrc, loff + K * 4, rowoff) : 0.0f;
+			});
+		}
+
+		// forward: local maps, scan, re-run with the carry
+		ACC A[S], B[S];
+		static_for<S>([&](auto s) {
+			A[s] = (ACC)1;
+			B[s] = (ACC)0;
+		});
+		static_for<C>([&](auto K) {
+			constexpr int s = K % S;
+			B[s] = fa[K] * B[s] + (ACC)v[K];
+			A[s] = fa[K] * A[s];
+		});
+		static_for<S>([&](auto s) {
+			scan_up(A[s], B[s]);
+			// carry into this lane = inclusive result of the lane before (0 for lane 0)
+			B[s] = dpp_mov<0x138, 0xf>((ACC)0, B[s]);
+		});
+		// forward results overwrite the loaded values (FP32, as the register
+		// kernels keep theirs)
+		static_for<C>([&](auto K) {
+			constexpr int s = K % S;
+			B[s] = fa[K] * B[s] + (ACC)v[K];
+			v[K] = (float)B[s];
+		});
+
+		// backward
+		static_for<S>([&](auto s) {
+			A[s] = (ACC)1;
+			B[s] = (ACC)0;
+		});
+		static_for<C>([&](auto K) {
+			constexpr int j = VPL - 1 - K / S, s = K % S;
+			constexpr int e = j * S + s;
+			const ACC cb = cs[s] * fbv[e];
+			B[s] = cb * B[s] + fbv[e] * (ACC)v[e];
+			A[s] = cb * A[s];
+		});
+		static_for<S>([&](auto s) {
+			scan_down(A[s], B[s], lane);
+			// carry from the right = inclusive result of the next lane (0 past lane 63)
+			B[s] = dpp_mov<0x130, 0xf>((ACC)0, B[s]);
+		});
+		static_for<C>([&](auto K) {
+			constexpr int j = VPL - 1 - K / S, s = K % S;
+			constexpr int e = j * S + s;
+			B[s] = cs[s] * fbv[e] * B[s] + fbv[e] * (ACC)v[e];
+			v[e] = (float)B[s];
+		});
+
+		if (vec && full)
+		{
+			static_for<C / 4>([&](auto Q) {
+				u32x4 w;
+				w.x = __builtin_bit_cast(unsigned, v[4 * Q + 0]);
+				w.y = __builtin_bit_cast(unsigned, v[4 * Q + 1]);
+				w.z = __builtin_bit_cast(unsigned, v[4 * Q + 2]);
+				w.w = __builtin_bit_cast(unsigned, v[4 * Q + 3]);
+				__builtin_amdgcn_raw_buffer_store_b128(w, rsrc, loff + Q * 16, rowoff, 0);
+			});
+		}
+		else
+		{
+			static_for<C>([&](auto K) {
+				constexpr int j = K / S;
+				if (lane * VPL + j < nx)
+					buf_store(rsrc, loff + K * 4, rowoff, v[K]);
+			});
+		}
+	}
+}
+
+} // namespace sweep
+} // namespace bfvm

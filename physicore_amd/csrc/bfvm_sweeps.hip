@@ -11,6 +11,7 @@ kernel_fn pick_cols_z64(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
+kernel_fn pick_xscan(int S, int vpl, bool fp64);
 
 namespace {
 template <typename ACC>
@@ -23,8 +24,10 @@ kernel_fn pick_lds(int axis)
 	return sweep_lds_kernel<2, ACC>;
 }
 
-kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces)
+kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 {
+	if (variant == kSweepScan)
+		return axis == 0 ? pick_xscan(S, nmax, fp64) : nullptr;
 	if (variant == kSweepLds)
 		return fp64 ? pick_lds<double>(axis) : pick_lds<float>(axis);
 	if (axis == 0)
@@ -46,6 +49,8 @@ int reg_bucket(int n)
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 {
 	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
+	if (variant == kSweepScan)
+		return 0;
 	if (axis == 0)
 		return fac + (size_t)x_rows_per_tile(S) * x_row_pitch(L, S) * sizeof(float);
 	if (variant == kSweepLds)
@@ -61,7 +66,7 @@ hipError_t init_sweep_kernels()
 			for (int p = 0; p < 4; p++)
 			{
 				const int variant = nmax == 0 ? kSweepLds : kSweepReg;
-				sweep::kernel_fn k = sweep::pick(axis, variant, nmax, p & 1, p & 2);
+				sweep::kernel_fn k = sweep::pick(axis, variant, nmax, p & 1, p & 2, 1);
 				if (!k)
 					return hipErrorInvalidValue;
 				hipError_t e =
@@ -72,7 +77,7 @@ hipError_t init_sweep_kernels()
 	return hipSuccess;
 }
 
-hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, const DomainDev* doms,
+hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, int S, const DomainDev* doms,
 						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
 						hipStream_t stream)
 {
@@ -80,7 +85,7 @@ hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, 
 		return hipSuccess;
 	if (grid <= 0 || grid > ntiles)
 		grid = ntiles;
-	sweep::kernel_fn k = sweep::pick(axis, variant, nmax, fp64, faces);
+	sweep::kernel_fn k = sweep::pick(axis, variant, nmax, fp64, faces, S);
 	if (!k)
 		return hipErrorInvalidValue;
 	hipLaunchKernelGGL(k, dim3(grid), dim3(kSweepThreads), lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap);

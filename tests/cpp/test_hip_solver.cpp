@@ -638,6 +638,7 @@ void conflict_case(microenvironment& m, bool internalized, bool recompute)
 	auto& data = dynamic_cast<agent_container&>(*m.agents).agent_data_ref();
 	const index_t S = m.substrates_count, n = data.agents_count;
 	std::vector<real_t> exp_int(n * S, 0);
+	m.solver->initialize(m); // the expectations read the initial densities
 	for (int step = 0; step < 2; step++)
 	{
 		auto want = expected_densities_1d(m);

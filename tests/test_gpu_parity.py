@@ -53,12 +53,20 @@ SHAPES = [
     (3, (50, 50, 50), 1), (3, (13, 70, 5), 5), (3, (40, 6, 31), 7), (3, (16, 16, 16), 8),
     (3, (1, 9, 12), 2), (3, (9, 1, 12), 3), (3, (9, 12, 1), 2), (2, (37, 29, 1), 3),
     (1, (101, 1, 1), 2), (3, (129, 3, 4), 1), (3, (3, 130, 5), 2), (3, (4, 5, 200), 4),
+    (3, (200, 6, 5), 4), (3, (257, 3, 2), 2), (3, (300, 4, 3), 1), (3, (77, 5, 4), 6),
+    (3, (6, 300, 3), 2), (3, (5, 4, 280), 3), (3, (130, 7, 3), 8), (1, (600, 1, 1), 3),
 ]
+
+
+# kernel variants: default (x by affine scan, y/z in registers), x through an
+# LDS transpose with register lines, and every line solved inside an LDS tile
+VARIANTS = {"default": {}, "xrows": {"no_xscan": 1}, "lds": {"force_lds_sweeps": 1}}
 
 
 @pytest.mark.parametrize("dims,grid,S", SHAPES)
 @pytest.mark.parametrize("quirks", [False, True])
-def test_diffusion_step_matches_oracle(dims, grid, S, quirks):
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_diffusion_step_matches_oracle(dims, grid, S, quirks, variant):
     rng = np.random.default_rng(hash((dims, grid, S)) % 2**32)
     D = 10 ** rng.uniform(0, 5, S)
     lam = rng.uniform(0.01, 1, S)
@@ -66,6 +74,8 @@ def test_diffusion_step_matches_oracle(dims, grid, S, quirks):
     f0 = rng.uniform(0, 1, p.shape)
     want = p.diffuse(f0.astype(np.float32).astype(np.float64))
     d = gpu_domain(p, field=f0)
+    if VARIANTS[variant]:
+        d.set_tuning(**VARIANTS[variant])
     d.step(1)
     err = rel_linf(d.field(), want)
     assert err.max() < 2e-6, err
@@ -245,7 +255,8 @@ def test_cells_conflict_match_oracle(builder, reduce, internalized):
         d.step(1)
         assert rel_linf(d.field(), f_or).max() < 1e-6
         if internalized:
-            assert rel_linf(d.internalized(), ag_or.internalized).max() < 1e-9
+            # the GPU reads D_old from the FP32 field, the oracle from FP64
+            assert rel_linf(d.internalized(), ag_or.internalized).max() < 1e-6
     for i in range(ag.n):
         s.release(i)
         d.release_internalized(i)
@@ -382,8 +393,9 @@ def test_graph_replay_matches_eager():
     """A 10-step hipGraph replayed several times, with eager work and syncs in
     between, equals the same steps launched eagerly."""
     p, f0, dirs, bulk, ag = sc.config2(side=32, S=2, n_cells=2000, seed=9)
-    a = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True)
-    b = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True)
+    # the sorted reduction makes both domains' cell sums bit-identical
+    a = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True, reduce="sorted")
+    b = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=True, reduce="sorted")
     for _ in range(3):
         a.step(10)
         b.step_graph(10, 10)
