@@ -150,7 +150,11 @@ int bfvm_download_internalized(bfvm_ctx* ctx, double* host);
 
 /* cell_solver::release_internalized_substrates (openmp_solver/src/cell_solver.cpp:351-374):
  * D += I*F/V at the agent's voxel, then I = 0.  No-op unless internalized
- * substrates are tracked. */
+ * substrates are tracked.  Releases are logged and applied by the next call
+ * that reads or writes the field or the internalized amounts (step, sync,
+ * transfers, mass, device pointer/stream, set_agents): per voxel, in call
+ * order, summed in FP64 and rounded to the FP32 field once, as the reference's
+ * FP64 field accumulates them. */
 int bfvm_release_internalized(bfvm_ctx* ctx, int64_t agent_index);
 
 /* Total amount per substrate, sum over voxels of density * voxel volume,
@@ -174,17 +178,26 @@ int bfvm_stream(bfvm_ctx* ctx, void** stream);
  * (0 when a stage does not run). */
 int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 
-/* Launch parameters of the sweep kernels.  persistent_wgs_per_cu = 0 launches
- * one workgroup per tile; > 0 launches that many workgroups per CU, each
- * looping over tiles.  force_lds_sweeps = 1 solves every line inside an LDS
- * tile instead of in registers (the variant used for lines longer than 256).
- * no_xscan = 1 solves x lines through an LDS transpose instead of by the
- * lane-parallel affine scan. */
+/* Kernel of the x sweep (lines of up to 256 voxels; longer lines are solved
+ * inside LDS tiles whatever is chosen). */
+typedef enum bfvm_x_sweep
+{
+	BFVM_X_AUTO = 0,	  /* the library's default for the shape */
+	BFVM_X_STRIDED = 1,	  /* one line per lane in VGPRs, read from HBM with stride S */
+	BFVM_X_TRANSPOSE = 2, /* rows staged in LDS, one line per lane in VGPRs */
+	BFVM_X_SCAN = 3		  /* lanes along the row, lane-parallel affine scans */
+} bfvm_x_sweep;
+
+/* Launch parameters of the sweep kernels.  persistent_wgs_per_cu: 0 = the
+ * default per kernel, -1 = one workgroup per tile, > 0 = that many workgroups
+ * per CU, each looping over tiles.  force_lds_sweeps = 1 solves every line
+ * inside an LDS tile instead of in registers (the variant used for lines
+ * longer than 256).  x_sweep: a bfvm_x_sweep value. */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
 	int32_t force_lds_sweeps;
-	int32_t no_xscan;
+	int32_t x_sweep;
 	int32_t reserved[5];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);

@@ -42,7 +42,7 @@ class DomainDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("force_lds_sweeps", C.c_int32),
-                ("no_xscan", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("x_sweep", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 _LIB = None
@@ -265,11 +265,11 @@ class Domain:
         t = Tuning()
         self._check(lib().bfvm_get_tuning(self._h, C.byref(t)))
         return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu,
-                "force_lds_sweeps": t.force_lds_sweeps, "no_xscan": t.no_xscan}
+                "force_lds_sweeps": t.force_lds_sweeps, "x_sweep": t.x_sweep}
 
-    def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0, no_xscan=0):
+    def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0, x_sweep=0):
         t = Tuning()
         t.persistent_wgs_per_cu = int(persistent_wgs_per_cu)
         t.force_lds_sweeps = int(force_lds_sweeps)
-        t.no_xscan = int(no_xscan)
+        t.x_sweep = int(x_sweep)
         self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))

@@ -37,7 +37,8 @@ def run(cmd, cwd=None):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
-HIP_SOURCES = ["bfvm_sweeps_y64.hip", "bfvm_sweeps_z64.hip", "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",
+HIP_SOURCES = ["bfvm_sweeps_x64.hip", "bfvm_sweeps_y64.hip", "bfvm_sweeps_z64.hip", "bfvm_sweeps_x32.hip",
+               "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",
                "bfvm_sweeps_rows.hip", "bfvm_sweeps_xscan.hip", "bfvm_sweeps.hip", "bfvm_kernels.hip",
                "bfvm_context.hip"]
 

@@ -12,6 +12,13 @@
 #include "bfvm_hip.h"
 #include "bfvm_internal.h"
 
+namespace {
+// x kernel of BFVM_X_AUTO: strided register lines (on 200^3 x 4, FP64
+// recurrences: 77 us, against 81 us for the scan with 2 workgroups per CU and
+// 99 us for the LDS transpose)
+constexpr int kDefaultXSweep = BFVM_X_STRIDED;
+} // namespace
+
 using namespace bfvm;
 
 namespace {
@@ -163,6 +170,9 @@ struct bfvm_ctx
 	bool recompute_pending = false;
 	int64_t n_owners = 0;
 	bool conflict = false;
+	// releases not yet applied to the field: (agent, voxel) in call order
+	std::vector<int64_t> rel_agent, rel_voxel;
+	dvec<int64_t> rel_dev;
 
 	dvec<TileRef> tiles[3];
 	int ntiles[3] = { 0, 0, 0 };
@@ -287,7 +297,7 @@ int sync_domain(bfvm_ctx* c)
 }
 
 // Tiles of one axis for one domain (dom index `dom` of a launch).
-void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, std::vector<TileRef>& t)
+void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, int fac_cap, std::vector<TileRef>& t)
 {
 	const int L = nx * S;
 	if (axis == 0 && variant == kSweepScan)
@@ -297,9 +307,9 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 			t.push_back({ dom, r0, std::min(kScanRowsPerTile, rows - r0), 0 });
 		return;
 	}
-	if (axis == 0)
+	if (axis == 0 && variant != kSweepReg)
 	{
-		const int R = x_rows_per_tile(S);
+		const int R = std::max(1, x_rows_per_tile(S, L, fac_cap));
 		const int rows = ny * nz;
 		for (int r0 = 0; r0 < rows; r0 += R)
 			t.push_back({ dom, r0, std::min(R, rows - r0), 0 });
@@ -308,8 +318,10 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 	const int outer = axis == 1 ? nz : ny;
 	if (variant == kSweepReg)
 	{
-		for (int f0 = 0; f0 < outer * L; f0 += kColsPerTile)
-			t.push_back({ dom, f0, 0, 0 });
+		// flattened (outer, column) space: x columns are the S lines of a row
+		const int64_t total = axis == 0 ? (int64_t)ny * nz * S : (int64_t)outer * L;
+		for (int64_t f0 = 0; f0 < total; f0 += kColsPerTile)
+			t.push_back({ dom, (int)f0, 0, 0 });
 		return;
 	}
 	for (int o = 0; o < outer; o++)
@@ -329,23 +341,32 @@ int build_tiles(bfvm_ctx* c)
 			const int n = (int)c->n[axis];
 			const int bucket = reg_bucket(n);
 			const int vpl = xscan_vpl(n, c->S);
-			// x: the scan kernel unless faces must be applied in this sweep (1-D domains)
-			if (axis == 0 && vpl && !(c->dims == 1 && c->face_mask) && !c->tuning.force_lds_sweeps &&
-				!c->tuning.no_xscan)
+			const int xs = c->tuning.x_sweep == BFVM_X_AUTO ? kDefaultXSweep : c->tuning.x_sweep;
+			if (c->tuning.force_lds_sweeps)
+				;
+			// the scan kernel does not apply faces, which 1-D domains need in this sweep
+			else if (axis == 0 && xs == BFVM_X_SCAN && vpl && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepScan;
 				c->nmax[axis] = vpl;
 			}
-			else if (bucket && !c->tuning.force_lds_sweeps)
+			else if (axis == 0 && xs == BFVM_X_TRANSPOSE && bucket)
+			{
+				c->variant[axis] = kSweepRows;
+				c->nmax[axis] = bucket;
+			}
+			else if (bucket)
 			{
 				c->variant[axis] = kSweepReg;
 				c->nmax[axis] = bucket;
 			}
-			domain_tiles(axis, c->variant[axis], 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, t);
 			// register kernels pad the factor rows to their bucket
-			c->fac_cap[axis] = (c->variant[axis] == kSweepReg ? c->nmax[axis] : n) * c->S;
+			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows;
+			c->fac_cap[axis] = (padded ? c->nmax[axis] : n) * c->S;
+			domain_tiles(axis, c->variant[axis], 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S,
+						 c->fac_cap[axis], t);
 			c->lds[axis] = sweep_lds_bytes(axis, c->variant[axis], n, c->S, c->L, c->fac_cap[axis]);
-			if (c->lds[axis] > 160 * 1024)
+			if (c->lds[axis] > (size_t)kMaxLdsBytes)
 				return c->fail(BFVM_ERR_UNSUPPORTED, "grid axis " + std::to_string(axis) + " of length " +
 														 std::to_string(n) + " needs more than 160 KiB of LDS per tile");
 		}
@@ -357,8 +378,13 @@ int build_tiles(bfvm_ctx* c)
 
 int sweep_grid(const bfvm_ctx* c, int axis)
 {
-	if (c->tuning.persistent_wgs_per_cu > 0)
-		return std::min(c->ntiles[axis], c->num_cu * c->tuning.persistent_wgs_per_cu);
+	int per_cu = c->tuning.persistent_wgs_per_cu;
+	// default: the scan kernel loops (2 workgroups per CU), which keeps its
+	// factors in registers across rows; the others take one workgroup per tile
+	if (per_cu == 0)
+		per_cu = c->variant[axis] == kSweepScan ? 2 : -1;
+	if (per_cu > 0)
+		return std::min(c->ntiles[axis], c->num_cu * per_cu);
 	return c->ntiles[axis];
 }
 
@@ -523,7 +549,46 @@ std::vector<SparseRef> refs_for(int64_t count)
 	return r;
 }
 
+// Applies the logged releases: grouped by voxel (call order kept within a
+// voxel), summed in FP64 and rounded to the field once per voxel.
+int flush_releases(bfvm_ctx* c)
+{
+	const size_t n = c->rel_agent.size();
+	if (n == 0)
+		return BFVM_OK;
+	std::vector<size_t> order(n);
+	for (size_t k = 0; k < n; k++)
+		order[k] = k;
+	std::stable_sort(order.begin(), order.end(),
+					 [&](size_t a, size_t b) { return c->rel_voxel[a] < c->rel_voxel[b]; });
+	std::vector<int64_t> groups, agents(n);
+	for (size_t k = 0; k < n; k++)
+	{
+		agents[k] = c->rel_agent[order[k]];
+		const int64_t v = c->rel_voxel[order[k]];
+		if (k == 0 || v != groups[groups.size() - 3])
+			groups.insert(groups.end(), { v, (int64_t)k, 0 });
+		groups.back()++;
+	}
+	const int64_t n_groups = (int64_t)groups.size() / 3;
+	std::vector<int64_t> packed(groups);
+	packed.insert(packed.end(), agents.begin(), agents.end());
+	c->rel_agent.clear();
+	c->rel_voxel.clear();
+	TRY_HIP(c, c->rel_dev.upload(packed.data(), packed.size(), c->stream));
+	TRY_HIP(c, launch_release_flush(c->ddom.p, c->rel_dev.p, n_groups, c->rel_dev.p + 3 * n_groups, c->S, c->stream));
+	return BFVM_OK;
+}
+
 } // namespace
+
+#define FLUSH_RELEASES(c)               \
+	do                                  \
+	{                                   \
+		int rc_ = flush_releases(c);    \
+		if (rc_)                        \
+			return rc_;                 \
+	} while (0)
 
 extern "C" {
 
@@ -567,7 +632,7 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	c->init.assign(desc->initial_conditions, desc->initial_conditions + c->S);
 	c->tuning.persistent_wgs_per_cu = 0;
 	c->tuning.force_lds_sweeps = 0;
-	c->tuning.no_xscan = 0;
+	c->tuning.x_sweep = BFVM_X_AUTO;
 
 	auto bail = [&](int rc) {
 		g_error = c->err;
@@ -766,6 +831,7 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	if (n > 0 && (!positions || !secretion || !saturation || !uptake || !net_export || !fraction_released ||
 				  !volumes || !is_active))
 		return c->fail(BFVM_ERR_INVALID, "null agent array");
+	FLUSH_RELEASES(c); // pending releases belong to the old agent arrays
 	const size_t nS = (size_t)n * c->S;
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->n_agents = n;
@@ -821,6 +887,7 @@ int bfvm_step(bfvm_ctx* c, int64_t iterations)
 {
 	if (!c || iterations < 0)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	for (int64_t it = 0; it < iterations; it++)
 	{
 		if (c->recompute_pending)
@@ -840,6 +907,7 @@ int bfvm_step_graph(bfvm_ctx* c, int64_t iterations, int64_t graph_steps)
 {
 	if (!c || iterations < 0 || graph_steps < 1)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	if (iterations == 0)
 		return BFVM_OK;
 	if (c->recompute_pending)
@@ -886,6 +954,7 @@ int bfvm_sync(bfvm_ctx* c)
 {
 	if (!c)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	return BFVM_OK;
 }
@@ -894,6 +963,7 @@ int bfvm_download_field_f32(bfvm_ctx* c, float* host)
 {
 	if (!c || !host)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	TRY_HIP(c, hipMemcpyAsync(host, c->field.p, c->field.n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	return BFVM_OK;
@@ -903,6 +973,7 @@ int bfvm_upload_field_f32(bfvm_ctx* c, const float* host)
 {
 	if (!c || !host)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	TRY_HIP(c, hipMemcpyAsync(c->field.p, host, c->field.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	return BFVM_OK;
@@ -935,6 +1006,7 @@ int bfvm_download_internalized(bfvm_ctx* c, double* host)
 {
 	if (!c || (!host && c->n_agents))
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	if (c->n_agents == 0)
 		return BFVM_OK;
 	TRY_HIP(c, hipMemcpyAsync(host, c->internalized_d.p, (size_t)c->n_agents * c->S * sizeof(double),
@@ -954,7 +1026,9 @@ int bfvm_release_internalized(bfvm_ctx* c, int64_t agent)
 	int64_t v;
 	if (!voxel_of(c, c->h_positions.data() + agent * c->dims, &v))
 		return c->fail(BFVM_ERR_OUT_OF_GRID, "agent outside the mesh");
-	TRY_HIP(c, launch_release(c->ddom.p, agent, v, c->stream));
+	// applied in FP64 at the next call that reads or writes the field
+	c->rel_agent.push_back(agent);
+	c->rel_voxel.push_back(v);
 	return BFVM_OK;
 }
 
@@ -962,6 +1036,7 @@ int bfvm_total_mass(bfvm_ctx* c, double* out)
 {
 	if (!c || !out)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	TRY_HIP(c, c->mass_partial.alloc((size_t)mass_blocks(c->V) * c->S));
 	TRY_HIP(c, c->mass_out.alloc(c->S));
 	TRY_HIP(c, launch_mass(c->field.p, c->S, c->V, c->mass_partial.p, c->mass_out.p,
@@ -1021,6 +1096,7 @@ int bfvm_field_device_ptr(bfvm_ctx* c, void** ptr)
 {
 	if (!c || !ptr)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	*ptr = c->field.p;
 	return BFVM_OK;
 }
@@ -1029,6 +1105,7 @@ int bfvm_stream(bfvm_ctx* c, void** stream)
 {
 	if (!c || !stream)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	*stream = (void*)c->stream;
 	return BFVM_OK;
 }
@@ -1037,6 +1114,7 @@ int bfvm_time_stages(bfvm_ctx* c, int64_t steps, double* out_us)
 {
 	if (!c || !out_us || steps < 1)
 		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
 	if (c->recompute_pending)
 	{
 		int rc = recompute(c);
@@ -1087,12 +1165,12 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 {
 	if (!c || !t)
 		return BFVM_ERR_INVALID;
-	if (t->persistent_wgs_per_cu < 0 || t->persistent_wgs_per_cu > 64)
-		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [0, 64]");
+	if (t->persistent_wgs_per_cu < -1 || t->persistent_wgs_per_cu > 64)
+		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [-1, 64]");
 	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
-	if (t->no_xscan != 0 && t->no_xscan != 1)
-		return c->fail(BFVM_ERR_INVALID, "no_xscan must be 0 or 1");
+	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_SCAN)
+		return c->fail(BFVM_ERR_INVALID, "x_sweep must be one of the bfvm_x_sweep values");
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->tuning = *t;
 	int rc = build_tiles(c);

@@ -97,8 +97,9 @@ struct SparseRef
 enum SweepVariant
 {
 	kSweepLds = 0,
-	kSweepReg = 1,
-	kSweepScan = 2 // x only: lanes along the row, affine scans; nmax carries voxels per lane
+	kSweepReg = 1,	// one line per lane in VGPRs, read straight from HBM (x: stride S)
+	kSweepScan = 2, // x only: lanes along the row, affine scans; nmax carries voxels per lane
+	kSweepRows = 3	// x only: rows staged in LDS, one line per lane in VGPRs
 };
 constexpr int kScanRowsPerTile = kSweepThreads / 64; // one row per wave
 // voxels per lane of the x-scan kernel for a row of nx voxels and S substrates, 0 if unsupported
@@ -119,8 +120,19 @@ int reg_bucket(int n);						// 32..256, or 0 when n > kRegMaxN
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);
 // x tiles: 64/S rows; LDS row pitch Lp = L + pad with Lp = S (mod 64), so the
 // 64 lanes (r, s) reading element i of their lines hit distinct banks.
-__host__ __device__ inline int x_rows_per_tile(int S) { return S >= kLinesPerTile ? 1 : kLinesPerTile / S; }
 __host__ __device__ inline int x_row_pitch(int L, int S) { return L + ((S - L % 64) % 64 + 64) % 64; }
+constexpr int kMaxLdsBytes = 160 * 1024;
+// Rows of an x tile staged in LDS: 64/S (one lane per (row, substrate)), fewer
+// when the rows would not fit beside the factor tables (fac_cap entries each of
+// b', e, plus 64 c values, all FP64); 0 when not even one row fits.
+inline int x_rows_per_tile(int S, int L, int fac_cap)
+{
+	const long long fac = (2LL * fac_cap + 64) * (long long)sizeof(double);
+	const long long row = (long long)x_row_pitch(L, S) * (long long)sizeof(float);
+	const long long fit = (kMaxLdsBytes - fac) / row;
+	const int want = S >= kLinesPerTile ? 1 : kLinesPerTile / S;
+	return fit <= 0 ? 0 : (int)(fit < want ? fit : want);
+}
 
 // Raises the dynamic-LDS limit of the sweep kernels to 160 KiB (once per process).
 hipError_t init_sweep_kernels();
@@ -144,7 +156,8 @@ hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* cel
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
 hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);
 hipError_t launch_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream);
-hipError_t launch_release(const DomainDev* dom, int64_t agent, int64_t voxel, hipStream_t stream);
+hipError_t launch_release_flush(const DomainDev* dom, const int64_t* groups, int64_t n_groups, const int64_t* agents,
+								int S, hipStream_t stream);
 
 // cell recompute (single domain)
 hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream);

@@ -313,18 +313,31 @@ __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, co
 	d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[slot * S + s] * d.den[q] + d.fac[q]);
 }
 
-// release_internal (cell_solver.cpp:274-289): D += I*F/V, I = 0.
-__global__ void release_kernel(const DomainDev* __restrict__ dp, int64_t i, int64_t voxel)
+// release_internal (cell_solver.cpp:274-289): D += I*F/V, I = 0, for a batch
+// of releases grouped by voxel.  groups[3g..3g+2] = (voxel, first, count) into
+// `agents` (call order within a voxel).  The sum runs in FP64 from the stored
+// density, as the reference's FP64 field does, and is rounded to the FP32 field
+// once.  One thread per (voxel, substrate).
+__global__ void release_flush_kernel(const DomainDev* __restrict__ dp, const int64_t* __restrict__ groups,
+									 int64_t n_groups, const int64_t* __restrict__ agents)
 {
 	const DomainDev& d = *dp;
 	const int S = d.S;
-	for (int s = threadIdx.x; s < S; s += blockDim.x)
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n_groups * S)
+		return;
+	const int64_t g = t / S;
+	const int s = (int)(t - g * S);
+	const int64_t v = groups[3 * g], first = groups[3 * g + 1], count = groups[3 * g + 2];
+	float* D = d.field + v * S + s;
+	double acc = (double)*D;
+	for (int64_t j = first; j < first + count; j++)
 	{
-		const int64_t q = i * S + s;
-		float* D = d.field + voxel * S + s;
-		*D = (float)((double)*D + d.internalized[q] * d.fraction_released[q] / d.voxel_volume);
+		const int64_t q = agents[j] * S + s;
+		acc += d.internalized[q] * d.fraction_released[q] / d.voxel_volume;
 		d.internalized[q] = 0;
 	}
+	*D = (float)acc;
 }
 
 #pragma clang fp contract(on)
@@ -600,10 +613,10 @@ hipError_t launch_convert_f32_to_f64(const float* src, double* dst, int64_t n, h
 	return launch1d(f32_to_f64_kernel, n, 256, stream, src, dst, n);
 }
 
-hipError_t launch_release(const DomainDev* dom, int64_t agent, int64_t voxel, hipStream_t stream)
+hipError_t launch_release_flush(const DomainDev* dom, const int64_t* groups, int64_t n_groups, const int64_t* agents,
+								int S, hipStream_t stream)
 {
-	hipLaunchKernelGGL(release_kernel, dim3(1), dim3(64), 0, stream, dom, agent, voxel);
-	return hipGetLastError();
+	return launch1d(release_flush_kernel, n_groups * S, 256, stream, dom, groups, n_groups, agents);
 }
 
 hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream)

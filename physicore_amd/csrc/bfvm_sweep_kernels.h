@@ -92,6 +92,8 @@ __device__ __forceinline__ void load_factors(const DomainDev& d, int axis, int n
 constexpr int kBufferWord3 = 0x00020000; // gfx9 raw buffer, 32-bit elements
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+__device__ __forceinline__ float u32_as_float(unsigned u) { return __builtin_bit_cast(float, u); }
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const DomainDev& d)
 {
 	const uint64_t bytes = (uint64_t)d.nx * d.ny * d.nz * d.S * sizeof(float);
@@ -216,11 +218,13 @@ __device__ __forceinline__ float face_value(const line_faces& f, int k, int n, f
 	return f.hi ? f.hi_v : out;
 }
 
-// y (AXIS 1) and z (AXIS 2) sweeps, one line per lane in registers.  A tile is
-// 256 consecutive columns of the flattened (outer, x*S + s) space starting at
-// tr.a; outer is z for the y-sweep and y for the z-sweep.  For the z-sweep the
-// flattened columns are contiguous in memory; for the y-sweep they are
-// contiguous within each z plane.  Padding steps (i >= n) reload element n-1
+// Sweeps with one line per lane in registers.  A tile is 256 consecutive
+// columns of a flattened (outer, column) space starting at tr.a: for y (AXIS 1)
+// and z (AXIS 2) the columns are x*S + s and outer is z for the y-sweep and y
+// for the z-sweep (contiguous within each z plane, resp. in memory); for x
+// (AXIS 0) the columns are the S substrates of a row and outer is the row
+// y + ny*z, so a wave reads 64/S rows with stride S and the caches turn the
+// short stride into whole-line fetches.  Padding steps (i >= n) reload element n-1
 // and store to it before the real value, which is stored last.  FACES: the
 // launch finishes the step for domains whose last axis this is.
 template <int AXIS, int NMAX, typename ACC, bool FACES>
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 	{
 		const TileRef tr = tiles[t];
 		const DomainDev& d = doms[tr.dom];
-		const int n = AXIS == 1 ? d.ny : d.nz;
+		const int n = AXIS == 0 ? d.nx : (AXIS == 1 ? d.ny : d.nz);
 		if (tr.dom != cur)
 		{
 			__syncthreads();
@@ -247,15 +251,18 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 			cur = tr.dom;
 		}
 		const int S = d.S, L = d.L;
-		const int outer_n = AXIS == 1 ? d.nz : d.ny;
+		// AXIS 0: columns are (row, s), row = y + ny*z, S of them per row
+		const int width = AXIS == 0 ? S : L;
+		const int outer_n = AXIS == 0 ? d.ny * d.nz : (AXIS == 1 ? d.nz : d.ny);
 		const int f = tr.a + (int)threadIdx.x;
-		if (f >= outer_n * L)
+		if (f >= outer_n * width)
 			continue;
-		const int outer = f / L;
-		const int c = f - outer * L;
-		const int stride4 = (AXIS == 1 ? L : d.ny * L) * (int)sizeof(float);
-		const int voff = (AXIS == 1 ? outer * d.ny * L + c : f) * (int)sizeof(float);
-		const int cs = d.coef_map ? d.coef_map[c] : c % S;
+		const int outer = f / width;
+		const int c = f - outer * width;
+		const int stride4 = (AXIS == 0 ? S : (AXIS == 1 ? L : d.ny * L)) * (int)sizeof(float);
+		const int voff =
+			(AXIS == 0 ? outer * L + c : (AXIS == 1 ? outer * d.ny * L + c : f)) * (int)sizeof(float);
+		const int cs = AXIS == 0 ? c : (d.coef_map ? d.coef_map[c] : c % S);
 		const __amdgpu_buffer_rsrc_t rsrc = field_rsrc(d);
 
 		float v[NMAX];
@@ -272,8 +279,13 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 			line_faces lf {};
 			if (finish && d.face_mask && AXIS == d.dims - 1)
 			{
-				const int x = c / S, s = c - (c / S) * S;
-				lf = AXIS == 1 ? faces_of_line(d, 1, x, 0, outer, s) : faces_of_line(d, 2, x, outer, 0, s);
+				if constexpr (AXIS == 0)
+					lf = faces_of_line(d, 0, 0, outer % d.ny, outer / d.ny, c);
+				else
+				{
+					const int x = c / S, s = c - (c / S) * S;
+					lf = AXIS == 1 ? faces_of_line(d, 1, x, 0, outer, s) : faces_of_line(d, 2, x, outer, 0, s);
+				}
 			}
 			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], [&](int i, float val) {
 				const int k = min(i, sn);
@@ -621,7 +633,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const Domain
 	(void)fac_cap;
 	constexpr int C = S * VPL; // floats per lane
 	const int lane = threadIdx.x & 63;
-	const int wave = threadIdx.x >> 6;
+	// uniform per wave: keeps the row offset in an SGPR (buffer soffset)
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	int cur = -1;
 	int nx = 0;
 	ACC fa[C], fbv[C], cs[S];
@@ -659,10 +672,12 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const Domain
 		{
 			static_for<C / 4>([&](auto Q) {
 				const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rsrc, loff + Q * 16, rowoff, 0);
-				v[4 * Q + 0] = __builtin_bit_cast(float, w.x);
-				v[4 * Q + 1] = __builtin_bit_cast(float, w.y);
-				v[4 * Q + 2] = __builtin_bit_cast(float, w.z);
-				v[4 * Q + 3] = __builtin_bit_cast(float, w.w);
+				// element values, not element lvalues: __builtin_bit_cast of an
+				// ext-vector element reads from the start of the vector
+				v[4 * Q + 0] = u32_as_float(w.x);
+				v[4 * Q + 1] = u32_as_float(w.y);
+				v[4 * Q + 2] = u32_as_float(w.z);
+				v[4 * Q + 3] = u32_as_float(w.w);
 			});
 		}
 		else

@@ -4,8 +4,12 @@
 // (separate translation units so they compile in parallel).
 #include "bfvm_sweep_kernels.h"
 
+#include <algorithm>
+
 namespace bfvm {
 namespace sweep {
+kernel_fn pick_cols_x64(int nmax, bool faces);
+kernel_fn pick_cols_x32(int nmax, bool faces);
 kernel_fn pick_cols_y64(int nmax, bool faces);
 kernel_fn pick_cols_z64(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
@@ -30,8 +34,10 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		return axis == 0 ? pick_xscan(S, nmax, fp64) : nullptr;
 	if (variant == kSweepLds)
 		return fp64 ? pick_lds<double>(axis) : pick_lds<float>(axis);
+	if (variant == kSweepRows)
+		return axis == 0 ? pick_rows(nmax, fp64) : nullptr;
 	if (axis == 0)
-		return pick_rows(nmax, fp64);
+		return fp64 ? pick_cols_x64(nmax, faces) : pick_cols_x32(nmax, faces);
 	if (axis == 1)
 		return fp64 ? pick_cols_y64(nmax, faces) : pick_cols_y32(nmax, faces);
 	return fp64 ? pick_cols_z64(nmax, faces) : pick_cols_z32(nmax, faces);
@@ -51,8 +57,8 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
 	if (variant == kSweepScan)
 		return 0;
-	if (axis == 0)
-		return fac + (size_t)x_rows_per_tile(S) * x_row_pitch(L, S) * sizeof(float);
+	if (axis == 0 && (variant == kSweepRows || variant == kSweepLds))
+		return fac + (size_t)std::max(1, x_rows_per_tile(S, L, fac_cap)) * x_row_pitch(L, S) * sizeof(float);
 	if (variant == kSweepLds)
 		return fac + (size_t)n * kLinesPerTile * sizeof(float);
 	return fac;
@@ -60,12 +66,18 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 
 hipError_t init_sweep_kernels()
 {
-	const int max_lds = 160 * 1024;
+	const int max_lds = kMaxLdsBytes;
 	for (int axis = 0; axis < 3; axis++)
 		for (int nmax = 0; nmax <= kRegMaxN; nmax += 32)
-			for (int p = 0; p < 4; p++)
+			for (int p = 0; p < 8; p++)
 			{
-				const int variant = nmax == 0 ? kSweepLds : kSweepReg;
+				int variant = nmax == 0 ? kSweepLds : kSweepReg;
+				if (p & 4)
+				{
+					if (axis != 0 || nmax == 0)
+						continue;
+					variant = kSweepRows;
+				}
 				sweep::kernel_fn k = sweep::pick(axis, variant, nmax, p & 1, p & 2, 1);
 				if (!k)
 					return hipErrorInvalidValue;

@@ -1,0 +1,29 @@
+// Instantiations of the x-sweep register kernel (strided lines), FP32 arithmetic.
+#include "bfvm_sweep_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_cols_x32(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_cols_kernel<0, N, float, true> : sweep_cols_kernel<0, N, float, false>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(32)
+		BFVM_BUCKET(64)
+		BFVM_BUCKET(96)
+		BFVM_BUCKET(128)
+		BFVM_BUCKET(160)
+		BFVM_BUCKET(192)
+		BFVM_BUCKET(224)
+		BFVM_BUCKET(256)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

@@ -58,9 +58,10 @@ SHAPES = [
 ]
 
 
-# kernel variants: default (x by affine scan, y/z in registers), x through an
-# LDS transpose with register lines, and every line solved inside an LDS tile
-VARIANTS = {"default": {}, "xrows": {"no_xscan": 1}, "lds": {"force_lds_sweeps": 1}}
+# kernel variants: the default, each x kernel (strided register lines, LDS
+# transpose, affine scan), and every line solved inside an LDS tile
+VARIANTS = {"default": {}, "xstrided": {"x_sweep": 1}, "xrows": {"x_sweep": 2}, "xscan": {"x_sweep": 3},
+            "lds": {"force_lds_sweeps": 1}}
 
 
 @pytest.mark.parametrize("dims,grid,S", SHAPES)
@@ -250,17 +251,24 @@ def test_cells_conflict_match_oracle(builder, reduce, internalized):
     s = orc.Solver(q, f_or, agents=ag_or, compute_internalized=internalized)
     d = gpu_domain(q, field=sc.constant_field(q, [1, 1]), agents=ag, internalized=internalized,
                    reduce=reduce)
-    for _ in range(2):
+    steps, peak = 2, 1.0
+    for _ in range(steps):
         s.solve(1)
         d.step(1)
         assert rel_linf(d.field(), f_or).max() < 1e-6
+        peak = max(peak, float(np.abs(f_or).max()))
         if internalized:
             # the GPU reads D_old from the FP32 field, the oracle from FP64
             assert rel_linf(d.internalized(), ag_or.internalized).max() < 1e-6
     for i in range(ag.n):
         s.release(i)
         d.release_internalized(i)
-    assert rel_linf(d.field(), f_or).max() < 1e-6
+    # The FP64 oracle returns to the initial density exactly.  The FP32 field
+    # stores each intermediate density once per step (rounding <= 2^-24 * peak;
+    # the densities here reach ~1.2e3), and the internalized amounts are FP64
+    # differences of the unrounded values, so the release is off by at most
+    # that rounding per step.
+    assert np.abs(d.field() - f_or).max() <= 1e-6 + steps * peak * 2.0 ** -24
 
 
 def _random_agents(p, n, rng, clustered=False):
