@@ -185,20 +185,26 @@ typedef enum bfvm_x_sweep
 	BFVM_X_AUTO = 0,	  /* the library's default for the shape */
 	BFVM_X_STRIDED = 1,	  /* one line per lane in VGPRs, read from HBM with stride S */
 	BFVM_X_TRANSPOSE = 2, /* rows staged in LDS, one line per lane in VGPRs */
-	BFVM_X_SCAN = 3		  /* lanes along the row, lane-parallel affine scans */
+	BFVM_X_SCAN = 3,	  /* lanes along the row, lane-parallel affine scans */
+	BFVM_X_PIPELINED = 4  /* rows prefetched into LDS while the previous tile is solved
+						   * (FP64, S in {1, 2, 4, 8}, nx*S % 4 == 0) */
 } bfvm_x_sweep;
 
 /* Launch parameters of the sweep kernels.  persistent_wgs_per_cu: 0 = the
  * default per kernel, -1 = one workgroup per tile, > 0 = that many workgroups
  * per CU, each looping over tiles.  force_lds_sweeps = 1 solves every line
  * inside an LDS tile instead of in registers (the variant used for lines
- * longer than 256).  x_sweep: a bfvm_x_sweep value. */
+ * longer than 256).  x_sweep: a bfvm_x_sweep value.  no_pipeline = 1 turns
+ * off the pipelined register sweeps of y and z (FP64, lines up to 256,
+ * nx*S % 4 == 0), in which the next tile of 64 lines streams into LDS while
+ * the current one is solved. */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
 	int32_t force_lds_sweeps;
 	int32_t x_sweep;
-	int32_t reserved[5];
+	int32_t no_pipeline;
+	int32_t reserved[4];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
 int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);

@@ -180,6 +180,7 @@ struct bfvm_ctx
 	int variant[3] = { 0, 0, 0 };
 	int nmax[3] = { 0, 0, 0 };
 	size_t lds[3] = { 0, 0, 0 };
+	PipeGeometry pipe[3];
 	bfvm_tuning tuning {};
 
 	DomainDev hdom {};
@@ -297,6 +298,29 @@ int sync_domain(bfvm_ctx* c)
 }
 
 // Tiles of one axis for one domain (dom index `dom` of a launch).
+// Tiles of the pipelined sweeps for one domain, padded with idle tiles (a < 0)
+// to a multiple of the workgroup's waves so that no round spans two domains.
+void domain_pipe_tiles(int axis, int dom, int nx, int ny, int nz, int S, const PipeGeometry& g,
+					   std::vector<TileRef>& t)
+{
+	const int L = nx * S;
+	const size_t first = t.size();
+	if (axis == 0)
+	{
+		const int rows = ny * nz;
+		for (int r0 = 0; r0 < rows; r0 += g.x_rows)
+			t.push_back({ dom, r0, std::min(g.x_rows, rows - r0), 0 });
+	}
+	else
+	{
+		const int64_t total = (int64_t)(axis == 1 ? nz : ny) * L;
+		for (int64_t f0 = 0; f0 < total; f0 += 64)
+			t.push_back({ dom, (int)f0, 0, 0 });
+	}
+	while ((t.size() - first) % (size_t)g.waves)
+		t.push_back({ dom, -1, 0, 0 });
+}
+
 void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, int fac_cap, std::vector<TileRef>& t)
 {
 	const int L = nx * S;
@@ -342,8 +366,22 @@ int build_tiles(bfvm_ctx* c)
 			const int bucket = reg_bucket(n);
 			const int vpl = xscan_vpl(n, c->S);
 			const int xs = c->tuning.x_sweep == BFVM_X_AUTO ? kDefaultXSweep : c->tuning.x_sweep;
+			const int64_t lines = axis == 0 ? c->n[1] * c->n[2] * c->S : c->V / c->n[axis] * c->S;
+			const int pbucket = pipe_bucket(axis, n);
+			const PipeGeometry pg = pipe_geometry(axis, n, pbucket, c->S, c->L, lines);
+			c->pipe[axis] = PipeGeometry {};
 			if (c->tuning.force_lds_sweeps)
 				;
+			// y/z: pipelined register lines (FP64) wherever the shape fits; x
+			// only on request (its strided stores make it slower than the
+			// strided register kernel)
+			else if (c->fp64 && !c->tuning.no_pipeline && pbucket && pg.waves > 0 &&
+					 (axis != 0 || c->tuning.x_sweep == BFVM_X_PIPELINED))
+			{
+				c->variant[axis] = kSweepPipe;
+				c->nmax[axis] = pbucket;
+				c->pipe[axis] = pg;
+			}
 			// the scan kernel does not apply faces, which 1-D domains need in this sweep
 			else if (axis == 0 && xs == BFVM_X_SCAN && vpl && !(c->dims == 1 && c->face_mask))
 			{
@@ -361,11 +399,20 @@ int build_tiles(bfvm_ctx* c)
 				c->nmax[axis] = bucket;
 			}
 			// register kernels pad the factor rows to their bucket
-			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows;
+			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows ||
+								c->variant[axis] == kSweepPipe;
 			c->fac_cap[axis] = (padded ? c->nmax[axis] : n) * c->S;
-			domain_tiles(axis, c->variant[axis], 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S,
-						 c->fac_cap[axis], t);
-			c->lds[axis] = sweep_lds_bytes(axis, c->variant[axis], n, c->S, c->L, c->fac_cap[axis]);
+			if (c->variant[axis] == kSweepPipe)
+			{
+				domain_pipe_tiles(axis, 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, c->pipe[axis], t);
+				c->lds[axis] = c->pipe[axis].lds_bytes;
+			}
+			else
+			{
+				domain_tiles(axis, c->variant[axis], 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S,
+							 c->fac_cap[axis], t);
+				c->lds[axis] = sweep_lds_bytes(axis, c->variant[axis], n, c->S, c->L, c->fac_cap[axis]);
+			}
 			if (c->lds[axis] > (size_t)kMaxLdsBytes)
 				return c->fail(BFVM_ERR_UNSUPPORTED, "grid axis " + std::to_string(axis) + " of length " +
 														 std::to_string(n) + " needs more than 160 KiB of LDS per tile");
@@ -381,6 +428,12 @@ int sweep_grid(const bfvm_ctx* c, int axis)
 	int per_cu = c->tuning.persistent_wgs_per_cu;
 	// default: the scan kernel loops (2 workgroups per CU), which keeps its
 	// factors in registers across rows; the others take one workgroup per tile
+	if (c->variant[axis] == kSweepPipe)
+	{
+		// persistent: one workgroup per CU (the LDS holds one), each looping
+		const int rounds = c->ntiles[axis] / c->pipe[axis].waves;
+		return std::min(rounds, c->num_cu * std::max(per_cu, 1));
+	}
 	if (per_cu == 0)
 		per_cu = c->variant[axis] == kSweepScan ? 2 : -1;
 	if (per_cu > 0)
@@ -397,7 +450,11 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 	TRY_HIP(c, mark(0));
 	for (int axis = 0; axis < 3; axis++)
 	{
-		if (axis < c->dims)
+		if (axis < c->dims && c->variant[axis] == kSweepPipe)
+			TRY_HIP(c, launch_sweep_pipe(axis, c->nmax[axis], axis == c->dims - 1 && c->face_mask != 0, c->ddom.p,
+										 c->tiles[axis].p, c->ntiles[axis], 1, c->fac_cap[axis], c->pipe[axis],
+										 sweep_grid(c, axis), c->stream));
+		else if (axis < c->dims)
 			TRY_HIP(c, launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64,
 									axis == c->dims - 1 && c->face_mask != 0, c->S, c->ddom.p, c->tiles[axis].p,
 									c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
@@ -633,6 +690,7 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	c->tuning.persistent_wgs_per_cu = 0;
 	c->tuning.force_lds_sweeps = 0;
 	c->tuning.x_sweep = BFVM_X_AUTO;
+	c->tuning.no_pipeline = 0;
 
 	auto bail = [&](int rc) {
 		g_error = c->err;
@@ -1169,8 +1227,10 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [-1, 64]");
 	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
-	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_SCAN)
+	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_PIPELINED)
 		return c->fail(BFVM_ERR_INVALID, "x_sweep must be one of the bfvm_x_sweep values");
+	if (t->no_pipeline != 0 && t->no_pipeline != 1)
+		return c->fail(BFVM_ERR_INVALID, "no_pipeline must be 0 or 1");
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->tuning = *t;
 	int rc = build_tiles(c);

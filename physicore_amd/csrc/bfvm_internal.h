@@ -99,8 +99,24 @@ enum SweepVariant
 	kSweepLds = 0,
 	kSweepReg = 1,	// one line per lane in VGPRs, read straight from HBM (x: stride S)
 	kSweepScan = 2, // x only: lanes along the row, affine scans; nmax carries voxels per lane
-	kSweepRows = 3	// x only: rows staged in LDS, one line per lane in VGPRs
+	kSweepRows = 3, // x only: rows staged in LDS, one line per lane in VGPRs
+	kSweepPipe = 4	// register lines fed by buffer-to-LDS prefetch of the next tile (FP64)
 };
+
+// Geometry of the pipelined sweeps (kSweepPipe): the per-wave tile buffer in
+// floats, the waves per workgroup that fit the LDS beside the b' table, and
+// the x rows per tile.  waves == 0: the shape does not fit (or L % 4 != 0).
+struct PipeGeometry
+{
+	int buf_floats = 0;
+	int waves = 0;
+	int x_rows = 0;
+	int S = 0;
+	size_t lds_bytes = 0;
+};
+// Register bucket of the pipelined kernels: 8 lines for y/z, 32 for x.
+int pipe_bucket(int axis, int n);
+PipeGeometry pipe_geometry(int axis, int n, int nmax, int S, int L, int64_t lines_in_tiles);
 constexpr int kScanRowsPerTile = kSweepThreads / 64; // one row per wave
 // voxels per lane of the x-scan kernel for a row of nx voxels and S substrates, 0 if unsupported
 __host__ inline int xscan_vpl(int nx, int S)
@@ -122,6 +138,14 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);
 // 64 lanes (r, s) reading element i of their lines hit distinct banks.
 __host__ __device__ inline int x_row_pitch(int L, int S) { return L + ((S - L % 64) % 64 + 64) % 64; }
 constexpr int kMaxLdsBytes = 160 * 1024;
+// Row pitch of the pipelined x tiles: a multiple of 4 floats (the buffer-to-LDS
+// loads write 16 bytes per lane) congruent to 4*ceil(S/4) mod 64, so the lanes
+// (row, substrate) reading their lines hit distinct banks when S % 4 == 0.
+__host__ __device__ inline int pipe_x_pitch(int L, int S)
+{
+	const int T = 4 * ((S + 3) / 4) % 64;
+	return L + ((T - L % 64) % 64 + 64) % 64;
+}
 // Rows of an x tile staged in LDS: 64/S (one lane per (row, substrate)), fewer
 // when the rows would not fit beside the factor tables (fac_cap entries each of
 // b', e, plus 64 c values, all FP64); 0 when not even one row fits.
@@ -136,6 +160,8 @@ inline int x_rows_per_tile(int S, int L, int fac_cap)
 
 // Raises the dynamic-LDS limit of the sweep kernels to 160 KiB (once per process).
 hipError_t init_sweep_kernels();
+hipError_t launch_sweep_pipe(int axis, int nmax, bool faces, const DomainDev* doms, const TileRef* tiles, int ntiles,
+							 int finish, int fac_cap, const PipeGeometry& g, int grid, hipStream_t stream);
 
 // ---- launchers (enqueue on `stream`, return hipError_t) ----
 // fac_cap: doubles reserved per factor table in LDS (max n*S over the domains).

@@ -1,0 +1,25 @@
+// Instantiations of the pipelined y-sweep (FP64 recurrences), lines of
+// 201..224 voxels in buckets of 8.
+#include "bfvm_pipe_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+pipe_fn pick_pipe_y_4(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_pipe_kernel<1, N, true, 0> : sweep_pipe_kernel<1, N, false, 0>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(208)
+		BFVM_BUCKET(216)
+		BFVM_BUCKET(224)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

@@ -2,7 +2,7 @@
 // variant, line bucket, precision, faces) and the LDS it needs.  The register
 // kernels are instantiated in bfvm_sweeps_{y,z}{64,32}.hip and bfvm_sweeps_rows.hip
 // (separate translation units so they compile in parallel).
-#include "bfvm_sweep_kernels.h"
+#include "bfvm_pipe_kernels.h"
 
 #include <algorithm>
 
@@ -16,6 +16,59 @@ kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
 kernel_fn pick_xscan(int S, int vpl, bool fp64);
+pipe_fn pick_pipe_x_s1(int nmax, bool faces);
+pipe_fn pick_pipe_x_s2(int nmax, bool faces);
+pipe_fn pick_pipe_x_s4(int nmax, bool faces);
+pipe_fn pick_pipe_x_s8(int nmax, bool faces);
+
+// substrate counts the pipelined x kernel is built for (its LDS reads step by S)
+bool pipe_x_supports(int S) { return S == 1 || S == 2 || S == 4 || S == 8; }
+
+namespace {
+pipe_fn pick_pipe_x(int nmax, bool faces, int S)
+{
+	switch (S)
+	{
+	case 1:
+		return pick_pipe_x_s1(nmax, faces);
+	case 2:
+		return pick_pipe_x_s2(nmax, faces);
+	case 4:
+		return pick_pipe_x_s4(nmax, faces);
+	case 8:
+		return pick_pipe_x_s8(nmax, faces);
+	default:
+		return nullptr;
+	}
+}
+} // namespace
+pipe_fn pick_pipe_y_0(int nmax, bool faces);
+pipe_fn pick_pipe_y_1(int nmax, bool faces);
+pipe_fn pick_pipe_y_2(int nmax, bool faces);
+pipe_fn pick_pipe_y_3(int nmax, bool faces);
+pipe_fn pick_pipe_y_4(int nmax, bool faces);
+pipe_fn pick_pipe_y_5(int nmax, bool faces);
+pipe_fn pick_pipe_z_0(int nmax, bool faces);
+pipe_fn pick_pipe_z_1(int nmax, bool faces);
+pipe_fn pick_pipe_z_2(int nmax, bool faces);
+pipe_fn pick_pipe_z_3(int nmax, bool faces);
+pipe_fn pick_pipe_z_4(int nmax, bool faces);
+pipe_fn pick_pipe_z_5(int nmax, bool faces);
+
+namespace {
+pipe_fn pick_pipe(int axis, int nmax, bool faces, int S)
+{
+	if (axis == 0)
+		return pick_pipe_x(nmax, faces, S);
+	// instantiation groups: 8..64, 72..128, 136..168, 176..200, 208..224, 232..256
+	const int g = nmax <= 64 ? 0 : nmax <= 128 ? 1 : nmax <= 168 ? 2 : nmax <= 200 ? 3 : nmax <= 224 ? 4 : 5;
+	static pipe_fn (*const y[6])(int, bool) = { pick_pipe_y_0, pick_pipe_y_1, pick_pipe_y_2,
+												pick_pipe_y_3, pick_pipe_y_4, pick_pipe_y_5 };
+	static pipe_fn (*const z[6])(int, bool) = { pick_pipe_z_0, pick_pipe_z_1, pick_pipe_z_2,
+												pick_pipe_z_3, pick_pipe_z_4, pick_pipe_z_5 };
+	return axis == 1 ? y[g](nmax, faces) : z[g](nmax, faces);
+}
+} // namespace
 
 namespace {
 template <typename ACC>
@@ -86,7 +139,65 @@ hipError_t init_sweep_kernels()
 				if (e != hipSuccess)
 					return e;
 			}
+	for (int axis = 0; axis < 3; axis++)
+		for (int nmax = pipe_bucket(axis, 1); nmax <= kRegMaxN; nmax += pipe_bucket(axis, 1))
+			for (int faces = 0; faces < 2; faces++)
+				for (int S : { 1, 2, 4, 8 })
+			{
+				if (axis != 0 && S != 1)
+					continue;
+				sweep::pipe_fn k = sweep::pick_pipe(axis, nmax, faces, S);
+				if (!k)
+					return hipErrorInvalidValue;
+				hipError_t e =
+					hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+				if (e != hipSuccess)
+					return e;
+			}
 	return hipSuccess;
+}
+
+int pipe_bucket(int axis, int n)
+{
+	const int q = axis == 0 ? 32 : 8;
+	return n > kRegMaxN ? 0 : ((n + q - 1) / q) * q;
+}
+
+PipeGeometry pipe_geometry(int axis, int n, int nmax, int S, int L, int64_t lines_in_tiles)
+{
+	PipeGeometry g;
+	g.S = S;
+	if (nmax <= 0 || L % 4 != 0 || S > 64 || lines_in_tiles <= 0 || (axis == 0 && !sweep::pipe_x_supports(S)))
+		return g;
+	const size_t fac = (size_t)(nmax * S + 64) * sizeof(double);
+	if (axis == 0)
+	{
+		g.x_rows = 64 / S;
+		g.buf_floats = g.x_rows * pipe_x_pitch(L, S);
+	}
+	else
+		g.buf_floats = ((n + 3) / 4) * 4 * 64;
+	const size_t buf = (size_t)g.buf_floats * sizeof(float);
+	if (fac + buf > (size_t)kMaxLdsBytes)
+		return PipeGeometry {};
+	// up to 4 waves; no more than there are tiles
+	const int64_t tiles = (lines_in_tiles + 63) / 64;
+	g.waves = (int)std::min<int64_t>({ (int64_t)((kMaxLdsBytes - fac) / buf), 4, tiles });
+	g.lds_bytes = fac + (size_t)g.waves * buf;
+	return g;
+}
+
+hipError_t launch_sweep_pipe(int axis, int nmax, bool faces, const DomainDev* doms, const TileRef* tiles, int ntiles,
+							 int finish, int fac_cap, const PipeGeometry& g, int grid, hipStream_t stream)
+{
+	if (ntiles <= 0)
+		return hipSuccess;
+	sweep::pipe_fn k = sweep::pick_pipe(axis, nmax, faces, g.S);
+	if (!k || g.waves <= 0)
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k, dim3(grid), dim3(64 * g.waves), g.lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap,
+					   g.buf_floats);
+	return hipGetLastError();
 }
 
 hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, int S, const DomainDev* doms,

@@ -58,10 +58,11 @@ SHAPES = [
 ]
 
 
-# kernel variants: the default, each x kernel (strided register lines, LDS
-# transpose, affine scan), and every line solved inside an LDS tile
+# kernel variants: the default (pipelined register lines where the shape
+# allows), each explicit x kernel (strided register lines, LDS transpose,
+# affine scan), no pipelining, and every line solved inside an LDS tile
 VARIANTS = {"default": {}, "xstrided": {"x_sweep": 1}, "xrows": {"x_sweep": 2}, "xscan": {"x_sweep": 3},
-            "lds": {"force_lds_sweeps": 1}}
+            "xpipe": {"x_sweep": 4}, "nopipe": {"no_pipeline": 1}, "lds": {"force_lds_sweeps": 1}}
 
 
 @pytest.mark.parametrize("dims,grid,S", SHAPES)
