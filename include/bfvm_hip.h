@@ -209,6 +209,37 @@ typedef struct bfvm_tuning
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
 int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);
 
+/*
+ * Batches: several contexts (of any shapes, substrate counts and agent sets)
+ * stepped together.  Each sweep is one persistent launch per group of domains
+ * that share a kernel, over the concatenated work list of their line tiles;
+ * the boundary, bulk and cell stages launch once per substrate count.  Every
+ * domain follows the stage order of bfvm_step and gets the same results as
+ * stepped alone.  The contexts are borrowed (they must outlive the batch) and
+ * stay usable on their own: their streams are ordered before and after each
+ * batch step by events.  No counterpart in the reference, whose solvers step
+ * one microenvironment per call (openmp_solver.cpp:23-40); this is the
+ * ensemble form of that call.
+ */
+typedef struct bfvm_batch bfvm_batch;
+int bfvm_batch_create(bfvm_ctx* const* ctxs, int32_t count, bfvm_batch** out);
+void bfvm_batch_destroy(bfvm_batch* batch);
+const char* bfvm_batch_last_error(const bfvm_batch* batch);
+
+/* `iterations` steps of every domain; with graph_steps > 0 the loop replays a
+ * HIP graph of graph_steps steps (rebuilt when a domain changes). */
+int bfvm_batch_step(bfvm_batch* batch, int64_t iterations, int64_t graph_steps);
+int bfvm_batch_sync(bfvm_batch* batch);
+
+/* Launches per step: sweeps and the other stages (after any pending
+ * recompute). */
+int bfvm_batch_info(bfvm_batch* batch, int32_t* sweep_launches, int32_t* stage_launches);
+
+/* Times `steps` steps of the batch under each candidate tuning (x kernel,
+ * y/z pipelining, persistence), applies the fastest to every context and
+ * returns it in *chosen (may be NULL).  Advances the domains. */
+int bfvm_batch_autotune(bfvm_batch* batch, int64_t steps, bfvm_tuning* chosen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,13 @@ _SIGS = {
     "bfvm_time_stages": ([C.c_void_p, C.c_int64, f64p], C.c_int),
     "bfvm_get_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
     "bfvm_set_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
+    "bfvm_batch_create": ([C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
+    "bfvm_batch_destroy": ([C.c_void_p], None),
+    "bfvm_batch_last_error": ([C.c_void_p], C.c_char_p),
+    "bfvm_batch_step": ([C.c_void_p, C.c_int64, C.c_int64], C.c_int),
+    "bfvm_batch_sync": ([C.c_void_p], C.c_int),
+    "bfvm_batch_info": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "bfvm_batch_autotune": ([C.c_void_p, C.c_int64, C.POINTER(Tuning)], C.c_int),
 }
 
 
@@ -276,3 +283,53 @@ class Domain:
         t.x_sweep = int(x_sweep)
         t.no_pipeline = int(no_pipeline)
         self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))
+
+
+def _tuning_dict(t):
+    return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu, "force_lds_sweeps": t.force_lds_sweeps,
+            "x_sweep": t.x_sweep, "no_pipeline": t.no_pipeline}
+
+
+class Batch:
+    """Several Domains stepped together (a bfvm_batch).  The domains stay
+    usable on their own; they must outlive the batch."""
+
+    def __init__(self, domains):
+        self.domains = list(domains)
+        arr = (C.c_void_p * len(self.domains))(*[d._h for d in self.domains])
+        h = C.c_void_p()
+        rc = lib().bfvm_batch_create(arr, len(self.domains), C.byref(h))
+        if rc != OK:
+            raise BfvmError(rc, lib().bfvm_batch_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != OK:
+            raise BfvmError(rc, lib().bfvm_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bfvm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, iterations=1, graph_steps=0):
+        self._check(lib().bfvm_batch_step(self._h, int(iterations), int(graph_steps)))
+
+    def sync(self):
+        self._check(lib().bfvm_batch_sync(self._h))
+
+    def info(self):
+        a, b = C.c_int32(), C.c_int32()
+        self._check(lib().bfvm_batch_info(self._h, C.byref(a), C.byref(b)))
+        return {"sweep_launches": a.value, "stage_launches": b.value}
+
+    def autotune(self, steps=5):
+        t = Tuning()
+        self._check(lib().bfvm_batch_autotune(self._h, int(steps), C.byref(t)))
+        return _tuning_dict(t)

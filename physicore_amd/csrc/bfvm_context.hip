@@ -7,6 +7,9 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
+#include <memory>
+#include <tuple>
 #include <vector>
 
 #include "bfvm_hip.h"
@@ -647,6 +650,356 @@ int flush_releases(bfvm_ctx* c)
 			return rc_;                 \
 	} while (0)
 
+// ---------------------------------------------------------------- batches
+//
+// A batch steps several contexts together: each sweep is one launch per group
+// of domains that share the kernel (axis, variant, register bucket; for the
+// x kernels templated on S also S) over the concatenated tile lists, so many
+// small domains fill the GPU that one of them alone would leave idle.  The
+// boundary, bulk and cell stages launch once per substrate count.  Every
+// domain runs the same stage order as alone, so its arithmetic is identical.
+struct bfvm_batch
+{
+	std::string err;
+	std::vector<bfvm_ctx*> ctx;
+	hipStream_t stream = nullptr;
+	int num_cu = 256;
+	hipEvent_t ev_in = nullptr, ev_out = nullptr;
+
+	struct SweepLaunch
+	{
+		int axis = 0, variant = 0, nmax = 0, S = 0;
+		bool fp64 = true, faces = false;
+		int fac_cap = 0, grid = 0, ntiles = 0;
+		size_t lds = 0;
+		PipeGeometry pg;
+		std::vector<int> doms;
+		dvec<TileRef> tiles;
+	};
+	struct RefLaunch
+	{
+		int S = 0;
+		int64_t n = 0;
+		dvec<SparseRef> refs;
+	};
+	std::vector<std::unique_ptr<SweepLaunch>> sweeps;
+	std::unique_ptr<RefLaunch> dirichlet;
+	std::vector<std::unique_ptr<RefLaunch>> bulk, cells, internalized;
+	dvec<DomainDev> ddoms;
+	std::vector<uint64_t> signature;
+
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t gexec = nullptr;
+	int64_t graph_steps = 0;
+	bool graph_valid = false;
+
+	int fail(int code, const std::string& m)
+	{
+		err = m;
+		return code;
+	}
+	int hip_fail(hipError_t e, const char* what)
+	{
+		err = std::string(what) + ": " + hipGetErrorString(e);
+		return e == hipErrorOutOfMemory ? BFVM_ERR_NO_MEMORY : BFVM_ERR_HIP;
+	}
+	void drop_graph()
+	{
+		if (gexec)
+			(void)hipGraphExecDestroy(gexec);
+		if (graph)
+			(void)hipGraphDestroy(graph);
+		gexec = nullptr;
+		graph = nullptr;
+		graph_valid = false;
+	}
+	~bfvm_batch()
+	{
+		drop_graph();
+		if (ev_in)
+			(void)hipEventDestroy(ev_in);
+		if (ev_out)
+			(void)hipEventDestroy(ev_out);
+		if (stream)
+			(void)hipStreamDestroy(stream);
+	}
+};
+
+namespace {
+
+// what the batch layout depends on, per context
+std::vector<uint64_t> batch_signature(const bfvm_batch* b)
+{
+	std::vector<uint64_t> sig;
+	for (const bfvm_ctx* c : b->ctx)
+	{
+		sig.push_back(c->version);
+		sig.push_back((uint64_t)c->n_owners);
+		sig.push_back((uint64_t)c->n_agents);
+		sig.push_back(c->conflict ? 1 : 0);
+		sig.push_back((uint64_t)c->dir_count);
+		sig.push_back((uint64_t)c->bulk_count);
+	}
+	return sig;
+}
+
+// Brings every context up to date (pending releases and recomputes, on its
+// own stream) and rebuilds the launch lists when any of them changed.
+int batch_prepare(bfvm_batch* b)
+{
+	for (size_t i = 0; i < b->ctx.size(); i++)
+	{
+		bfvm_ctx* c = b->ctx[i];
+		int rc = flush_releases(c);
+		if (!rc && c->recompute_pending)
+			rc = recompute(c);
+		if (rc)
+			return b->fail(rc, "domain " + std::to_string(i) + ": " + c->err);
+	}
+	std::vector<uint64_t> sig = batch_signature(b);
+	if (sig == b->signature && b->ddoms.p)
+		return BFVM_OK;
+	b->drop_graph();
+	// launches in flight still read the tables about to be replaced
+	if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess)
+		return b->hip_fail(e, "hipStreamSynchronize");
+	for (bfvm_ctx* c : b->ctx)
+		if (hipError_t e = hipStreamSynchronize(c->stream); e != hipSuccess)
+			return b->hip_fail(e, "hipStreamSynchronize");
+
+	std::vector<DomainDev> h(b->ctx.size());
+	for (size_t i = 0; i < b->ctx.size(); i++)
+		h[i] = b->ctx[i]->hdom;
+	if (hipError_t e = b->ddoms.upload(h.data(), h.size(), b->stream); e != hipSuccess)
+		return b->hip_fail(e, "domain table upload");
+
+	// sweeps: group domains by kernel
+	b->sweeps.clear();
+	for (int axis = 0; axis < 3; axis++)
+	{
+		std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
+		for (size_t i = 0; i < b->ctx.size(); i++)
+		{
+			const bfvm_ctx* c = b->ctx[i];
+			if (axis >= c->dims)
+				continue;
+			const int v = c->variant[axis];
+			// kernels templated on S: the x scan, the pipelined x kernel
+			const int keyS = (v == kSweepScan || (v == kSweepPipe && axis == 0)) ? c->S : 0;
+			groups[{ v, c->nmax[axis], c->fp64 ? 1 : 0, keyS }].push_back((int)i);
+		}
+		for (auto& [key, doms] : groups)
+		{
+			auto L = std::make_unique<bfvm_batch::SweepLaunch>();
+			L->axis = axis;
+			L->variant = std::get<0>(key);
+			L->nmax = std::get<1>(key);
+			L->fp64 = std::get<2>(key) != 0;
+			L->doms = doms;
+			int maxS = 1;
+			int64_t lines = 0;
+			for (int i : doms)
+			{
+				const bfvm_ctx* c = b->ctx[i];
+				L->fac_cap = std::max(L->fac_cap, c->fac_cap[axis]);
+				L->faces |= axis == c->dims - 1 && c->face_mask != 0;
+				maxS = std::max(maxS, c->S);
+				lines += axis == 0 ? c->n[1] * c->n[2] * c->S : c->V / c->n[axis] * c->S;
+			}
+			L->S = maxS;
+			std::vector<TileRef> t;
+			if (L->variant == kSweepPipe)
+			{
+				// one geometry for the group: the largest tile buffer of its
+				// domains, the factor table of the most substrates
+				PipeGeometry g;
+				g.S = maxS;
+				for (int i : doms)
+				{
+					const bfvm_ctx* c = b->ctx[i];
+					g.buf_floats = std::max(g.buf_floats, c->pipe[axis].buf_floats);
+					g.x_rows = c->pipe[axis].x_rows;
+				}
+				const size_t fac = (size_t)(L->nmax * maxS + 64) * sizeof(double);
+				const size_t buf = (size_t)g.buf_floats * sizeof(float);
+				const int64_t tiles = (lines + 63) / 64;
+				g.waves = fac + buf > (size_t)kMaxLdsBytes
+							  ? 0
+							  : (int)std::min<int64_t>({ (int64_t)((kMaxLdsBytes - fac) / buf), 4, tiles });
+				if (g.waves <= 0)
+					return b->fail(BFVM_ERR_UNSUPPORTED, "pipelined sweep group does not fit the LDS");
+				g.lds_bytes = fac + (size_t)g.waves * buf;
+				L->pg = g;
+				L->fac_cap = L->nmax * maxS;
+				L->lds = g.lds_bytes;
+				for (int i : doms)
+				{
+					const bfvm_ctx* c = b->ctx[i];
+					domain_pipe_tiles(axis, i, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, g, t);
+				}
+				L->ntiles = (int)t.size();
+				L->grid = std::min(L->ntiles / g.waves, b->num_cu);
+			}
+			else
+			{
+				for (int i : doms)
+				{
+					const bfvm_ctx* c = b->ctx[i];
+					domain_tiles(axis, L->variant, i, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, L->fac_cap, t);
+					L->lds = std::max(L->lds, sweep_lds_bytes(axis, L->variant, (int)c->n[axis], c->S, c->L,
+															  L->fac_cap));
+				}
+				if (L->lds > (size_t)kMaxLdsBytes)
+					return b->fail(BFVM_ERR_UNSUPPORTED, "sweep group needs more than 160 KiB of LDS");
+				L->ntiles = (int)t.size();
+				L->grid = L->variant == kSweepScan ? std::min(L->ntiles, 2 * b->num_cu) : L->ntiles;
+			}
+			if (hipError_t e = L->tiles.upload(t.data(), t.size(), b->stream); e != hipSuccess)
+				return b->hip_fail(e, "tile upload");
+			b->sweeps.push_back(std::move(L));
+		}
+	}
+
+	// boundary, bulk and cells: refs (domain, entry), per substrate count
+	auto make_refs = [&](auto count_of, std::vector<std::unique_ptr<bfvm_batch::RefLaunch>>& out,
+						 bool by_S) -> int {
+		out.clear();
+		std::map<int, std::vector<SparseRef>> by;
+		for (size_t i = 0; i < b->ctx.size(); i++)
+		{
+			const int64_t n = count_of(b->ctx[i]);
+			auto& v = by[by_S ? b->ctx[i]->S : 0];
+			for (int64_t k = 0; k < n; k++)
+				v.push_back({ (int32_t)i, (int32_t)k });
+		}
+		for (auto& [S, v] : by)
+		{
+			if (v.empty())
+				continue;
+			auto R = std::make_unique<bfvm_batch::RefLaunch>();
+			R->S = S;
+			R->n = (int64_t)v.size();
+			if (hipError_t e = R->refs.upload(v.data(), v.size(), b->stream); e != hipSuccess)
+				return b->hip_fail(e, "ref upload");
+			out.push_back(std::move(R));
+		}
+		return BFVM_OK;
+	};
+	std::vector<std::unique_ptr<bfvm_batch::RefLaunch>> dir;
+	int rc = make_refs([](const bfvm_ctx* c) { return c->dir_count; }, dir, false);
+	if (!rc)
+		rc = make_refs([](const bfvm_ctx* c) { return c->bulk_count; }, b->bulk, true);
+	if (!rc)
+		rc = make_refs([](const bfvm_ctx* c) { return c->n_owners; }, b->cells, true);
+	if (!rc)
+		rc = make_refs([](const bfvm_ctx* c) { return c->internalized && c->conflict ? c->n_agents : 0; },
+					   b->internalized, true);
+	if (rc)
+		return rc;
+	b->dirichlet = dir.empty() ? nullptr : std::move(dir[0]);
+	b->signature = sig;
+	return BFVM_OK;
+}
+
+int batch_enqueue_step(bfvm_batch* b)
+{
+	for (auto& L : b->sweeps)
+	{
+		hipError_t e;
+		if (L->variant == kSweepPipe)
+			e = launch_sweep_pipe(L->axis, L->nmax, L->faces, b->ddoms.p, L->tiles.p, L->ntiles, 1, L->fac_cap, L->pg,
+								  L->grid, b->stream);
+		else
+			e = launch_sweep(L->axis, L->variant, L->nmax, L->fp64, L->faces, L->S, b->ddoms.p, L->tiles.p,
+							 L->ntiles, 1, L->fac_cap, L->lds, L->grid, b->stream);
+		if (e != hipSuccess)
+			return b->hip_fail(e, "sweep launch");
+	}
+	if (b->dirichlet)
+		if (hipError_t e = launch_dirichlet_interior(b->ddoms.p, b->dirichlet->refs.p, b->dirichlet->n, b->stream);
+			e != hipSuccess)
+			return b->hip_fail(e, "dirichlet launch");
+	for (auto& R : b->bulk)
+		if (hipError_t e = launch_bulk(b->ddoms.p, R->refs.p, R->n, R->S, b->stream); e != hipSuccess)
+			return b->hip_fail(e, "bulk launch");
+	for (auto& R : b->cells)
+		if (hipError_t e = launch_cells_apply(b->ddoms.p, R->refs.p, R->n, R->S, b->stream); e != hipSuccess)
+			return b->hip_fail(e, "cells launch");
+	for (auto& R : b->internalized)
+		if (hipError_t e = launch_cells_internalized(b->ddoms.p, R->refs.p, R->n, R->S, b->stream); e != hipSuccess)
+			return b->hip_fail(e, "internalized launch");
+	return BFVM_OK;
+}
+
+// the batch stream waits for each context's queued work; afterwards each
+// context stream waits for the batch
+int batch_join_in(bfvm_batch* b)
+{
+	for (bfvm_ctx* c : b->ctx)
+	{
+		if (hipError_t e = hipEventRecord(b->ev_in, c->stream); e != hipSuccess)
+			return b->hip_fail(e, "hipEventRecord");
+		if (hipError_t e = hipStreamWaitEvent(b->stream, b->ev_in, 0); e != hipSuccess)
+			return b->hip_fail(e, "hipStreamWaitEvent");
+	}
+	return BFVM_OK;
+}
+
+int batch_join_out(bfvm_batch* b)
+{
+	if (hipError_t e = hipEventRecord(b->ev_out, b->stream); e != hipSuccess)
+		return b->hip_fail(e, "hipEventRecord");
+	for (bfvm_ctx* c : b->ctx)
+		if (hipError_t e = hipStreamWaitEvent(c->stream, b->ev_out, 0); e != hipSuccess)
+			return b->hip_fail(e, "hipStreamWaitEvent");
+	return BFVM_OK;
+}
+
+int batch_run(bfvm_batch* b, int64_t iterations, int64_t graph_steps)
+{
+	int rc = batch_prepare(b);
+	if (rc)
+		return rc;
+	if ((rc = batch_join_in(b)))
+		return rc;
+	int64_t done = 0;
+	if (graph_steps > 0 && iterations >= graph_steps)
+	{
+		if (!b->graph_valid || b->graph_steps != graph_steps)
+		{
+			b->drop_graph();
+			if (hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal); e != hipSuccess)
+				return b->hip_fail(e, "hipStreamBeginCapture");
+			for (int64_t k = 0; k < graph_steps && rc == BFVM_OK; k++)
+				rc = batch_enqueue_step(b);
+			hipGraph_t g = nullptr;
+			hipError_t e = hipStreamEndCapture(b->stream, &g);
+			if (rc)
+			{
+				if (g)
+					(void)hipGraphDestroy(g);
+				return rc;
+			}
+			if (e != hipSuccess)
+				return b->hip_fail(e, "hipStreamEndCapture");
+			b->graph = g;
+			if ((e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+				return b->hip_fail(e, "hipGraphInstantiate");
+			b->graph_steps = graph_steps;
+			b->graph_valid = true;
+		}
+		for (; done + graph_steps <= iterations; done += graph_steps)
+			if (hipError_t e = hipGraphLaunch(b->gexec, b->stream); e != hipSuccess)
+				return b->hip_fail(e, "hipGraphLaunch");
+	}
+	for (; done < iterations; done++)
+		if ((rc = batch_enqueue_step(b)))
+			return rc;
+	return batch_join_out(b);
+}
+
+} // namespace
+
 extern "C" {
 
 const char* bfvm_last_error(const bfvm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
@@ -1238,6 +1591,160 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return rc;
 	c->version++;
 	return BFVM_OK;
+}
+
+
+int bfvm_batch_create(bfvm_ctx* const* ctxs, int32_t count, bfvm_batch** out)
+{
+	if (!out)
+		return BFVM_ERR_INVALID;
+	*out = nullptr;
+	if (!ctxs || count < 1)
+	{
+		g_error = "a batch needs at least one context";
+		return BFVM_ERR_INVALID;
+	}
+	auto b = std::make_unique<bfvm_batch>();
+	for (int32_t i = 0; i < count; i++)
+	{
+		if (!ctxs[i])
+		{
+			g_error = "null context in batch";
+			return BFVM_ERR_INVALID;
+		}
+		for (int32_t j = 0; j < i; j++)
+			if (ctxs[j] == ctxs[i])
+			{
+				g_error = "a context appears twice in the batch";
+				return BFVM_ERR_INVALID;
+			}
+		b->ctx.push_back(ctxs[i]);
+	}
+	b->num_cu = ctxs[0]->num_cu;
+	hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+	if (e == hipSuccess)
+		e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+	if (e == hipSuccess)
+		e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+	if (e != hipSuccess)
+	{
+		g_error = std::string("batch stream/events: ") + hipGetErrorString(e);
+		return BFVM_ERR_HIP;
+	}
+	*out = b.release();
+	return BFVM_OK;
+}
+
+void bfvm_batch_destroy(bfvm_batch* b)
+{
+	if (b && b->stream)
+		(void)hipStreamSynchronize(b->stream);
+	delete b;
+}
+
+const char* bfvm_batch_last_error(const bfvm_batch* b) { return b ? b->err.c_str() : g_error.c_str(); }
+
+int bfvm_batch_step(bfvm_batch* b, int64_t iterations, int64_t graph_steps)
+{
+	if (!b || iterations < 0 || graph_steps < 0)
+		return BFVM_ERR_INVALID;
+	if (iterations == 0)
+		return BFVM_OK;
+	return batch_run(b, iterations, graph_steps);
+}
+
+int bfvm_batch_sync(bfvm_batch* b)
+{
+	if (!b)
+		return BFVM_ERR_INVALID;
+	if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess)
+		return b->hip_fail(e, "hipStreamSynchronize");
+	return BFVM_OK;
+}
+
+int bfvm_batch_info(bfvm_batch* b, int32_t* sweep_launches, int32_t* stage_launches)
+{
+	if (!b)
+		return BFVM_ERR_INVALID;
+	int rc = batch_prepare(b);
+	if (rc)
+		return rc;
+	if (sweep_launches)
+		*sweep_launches = (int32_t)b->sweeps.size();
+	if (stage_launches)
+		*stage_launches = (int32_t)((b->dirichlet ? 1 : 0) + b->bulk.size() + b->cells.size() + b->internalized.size());
+	return BFVM_OK;
+}
+
+int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
+{
+	if (!b || steps < 1)
+		return BFVM_ERR_INVALID;
+	// candidate tunings, applied to every context of the batch: the x kernel,
+	// pipelining of y/z, persistence of the register kernels
+	std::vector<bfvm_tuning> cand;
+	auto T = [](int x, int nopipe, int per_cu) {
+		bfvm_tuning t {};
+		t.x_sweep = x;
+		t.no_pipeline = nopipe;
+		t.persistent_wgs_per_cu = per_cu;
+		return t;
+	};
+	cand.push_back(T(BFVM_X_AUTO, 0, 0));
+	cand.push_back(T(BFVM_X_SCAN, 0, 0));
+	cand.push_back(T(BFVM_X_TRANSPOSE, 0, 0));
+	cand.push_back(T(BFVM_X_PIPELINED, 0, 0));
+	cand.push_back(T(BFVM_X_AUTO, 1, 0));
+	cand.push_back(T(BFVM_X_AUTO, 1, 2));
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (hipError_t e = hipEventCreate(&e0); e != hipSuccess)
+		return b->hip_fail(e, "hipEventCreate");
+	if (hipError_t e = hipEventCreate(&e1); e != hipSuccess)
+	{
+		(void)hipEventDestroy(e0);
+		return b->hip_fail(e, "hipEventCreate");
+	}
+	int best = -1, rc = BFVM_OK;
+	float best_ms = 0;
+	for (size_t k = 0; k < cand.size() && rc == BFVM_OK; k++)
+	{
+		for (bfvm_ctx* c : b->ctx)
+			if ((rc = bfvm_set_tuning(c, &cand[k])))
+			{
+				b->err = c->err;
+				break;
+			}
+		if (rc)
+			break;
+		if ((rc = batch_run(b, 2, 0)))
+			break;
+		(void)hipEventRecord(e0, b->stream);
+		if ((rc = batch_run(b, steps, 0)))
+			break;
+		(void)hipEventRecord(e1, b->stream);
+		float ms = 0;
+		if (hipError_t e = hipEventSynchronize(e1); e != hipSuccess)
+		{
+			rc = b->hip_fail(e, "hipEventSynchronize");
+			break;
+		}
+		(void)hipEventElapsedTime(&ms, e0, e1);
+		if (best < 0 || ms < best_ms)
+		{
+			best = (int)k;
+			best_ms = ms;
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (rc)
+		return rc;
+	for (bfvm_ctx* c : b->ctx)
+		if ((rc = bfvm_set_tuning(c, &cand[best])))
+			return b->fail(rc, c->err);
+	if (chosen)
+		*chosen = cand[best];
+	return batch_prepare(b);
 }
 
 } // extern "C"

@@ -177,3 +177,58 @@ def config1(seed=42):
     ag.saturation[:] = 38.0
     ag.volumes[:] = 2494.0
     return p, field, dirs, None, ag
+
+
+def mixed_domain(rng, sides, max_S=8, max_cells=50_000, dims=3, extras=False, dx=20):
+    """One domain of the batched ensemble: each axis side drawn from `sides`
+    (anisotropic), 1..max_S substrates with D/decay as config 2, field U[0,1],
+    0..max_cells uniform cells (secretion/uptake U[0,10], saturation 1,
+    volume 2494).  extras adds what the ensemble itself does not exercise:
+    random Dirichlet faces, a few interior Dirichlet voxels, a bulk patch, net
+    export, internalized tracking and clustered (shared-voxel) cells."""
+    S = int(rng.integers(1, max_S + 1))
+    n = [int(rng.choice(sides)) if a < dims else 1 for a in range(3)]
+    D, lam = _d_lambda(rng, S)
+    p = orc.Problem(dims, tuple(n), (dx, dx, dx), 0.01, D, lam)
+    field = rng.uniform(0.0, 1.0, p.shape)
+    dirs = None
+    bulk = None
+    internalized = False
+    if extras:
+        dirs = orc.DirichletSpec(S, dims)
+        for a in range(dims):
+            for side in ("min", "max"):
+                if rng.uniform() < 0.3:
+                    dirs.set_face(a, side, rng.uniform(0, 2, S), (rng.uniform(0, 1, S) < 0.7).astype(np.uint8))
+        k = int(rng.integers(0, 4))
+        if k:
+            vox = np.stack([rng.integers(0, n[a], k) for a in range(dims)], axis=1)
+            dirs.set_interior(vox, rng.uniform(0, 2, (k, S)), np.ones((k, S), dtype=np.uint8))
+        if rng.uniform() < 0.6:
+            supply, uptake, target = (np.zeros(p.shape) for _ in range(3))
+            sl = tuple(slice(0, max(1, n[2 - a] // 3)) for a in range(3))
+            supply[sl] = rng.uniform(0, 1, S)
+            uptake[sl] = rng.uniform(0, 1, S)
+            target[sl] = rng.uniform(0, 2, S)
+            bulk = (supply, uptake, target)
+        internalized = bool(rng.uniform() < 0.5)
+    ncell = int(rng.integers(0, max_cells + 1))
+    ag = None
+    if ncell:
+        box = np.array([n[a] * dx for a in range(3)], dtype=np.float64)
+        ag = orc.AgentSpec(ncell, S, dims)
+        if extras and rng.uniform() < 0.5:
+            pos = rng.normal(box / 2, box / 8, (ncell, 3))
+        else:
+            pos = rng.uniform(0.0, box, (ncell, 3))
+        pos = np.clip(pos, 0, np.nextafter(box, 0))
+        ag.positions[:] = pos[:, :dims]
+        ag.secretion[:] = rng.uniform(0.0, 10.0, (ncell, S))
+        ag.uptake[:] = rng.uniform(0.0, 10.0, (ncell, S))
+        ag.saturation[:] = 1.0
+        ag.volumes[:] = 2494.0
+        if extras:
+            ag.net_export[:] = rng.uniform(-1.0, 1.0, (ncell, S))
+            ag.fraction_released[:] = rng.uniform(0.0, 1.0, (ncell, S))
+            ag.is_active[:] = rng.uniform(0, 1, ncell) < 0.9
+    return p, field, dirs, bulk, ag, internalized

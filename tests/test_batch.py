@@ -1,0 +1,149 @@
+"""Batched stepping (bfvm_batch_*): several domains of mixed shapes, substrate
+counts and agent sets stepped together must give each domain the result it
+gets when stepped alone, and the oracle's result."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import scenarios as sc
+from gpu_helpers import gpu_domain, rel_linf, require_gpu
+from physicore_amd import bfvm
+
+pytestmark = pytest.mark.gpu
+
+SMALL_SIDES = [4, 7, 8, 12, 16, 23, 32, 40]
+
+
+def _make(rng_seed, count, extras, sides=SMALL_SIDES, max_cells=400, dims_choice=(1, 2, 3)):
+    """count scenario tuples, each built from its own seed so the same domain
+    can be rebuilt exactly."""
+    out = []
+    for k in range(count):
+        rng = np.random.default_rng(rng_seed * 1000 + k)
+        dims = int(rng.choice(dims_choice))
+        out.append(sc.mixed_domain(rng, sides, max_S=8, max_cells=max_cells, dims=dims, extras=extras))
+    return out
+
+
+def _gpu(scn, reduce="sorted"):
+    p, field, dirs, bulk, ag, internalized = scn
+    return gpu_domain(p, field=field, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized,
+                      reduce=reduce)
+
+
+def test_batch_equals_alone_bit_exact():
+    require_gpu()
+    scns = _make(1, 14, extras=True)
+    alone = [_gpu(s) for s in scns]
+    mixed = [_gpu(s) for s in scns]
+    b = bfvm.Batch(mixed)
+    for d in alone:
+        d.step(7)
+    b.step(7)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a.field(), m.field()), k
+        if scns[k][5]:
+            assert np.array_equal(a.internalized(), m.internalized()), k
+
+
+def test_batch_graph_equals_eager():
+    require_gpu()
+    scns = _make(2, 9, extras=True)
+    eager = [_gpu(s) for s in scns]
+    graph = [_gpu(s) for s in scns]
+    be, bg = bfvm.Batch(eager), bfvm.Batch(graph)
+    be.step(23)
+    bg.step(23, graph_steps=10)  # two replays and three eager steps
+    for k, (a, g) in enumerate(zip(eager, graph)):
+        assert np.array_equal(a.field(), g.field()), k
+
+
+def test_batch_matches_oracle():
+    require_gpu()
+    scns = _make(3, 6, extras=True, max_cells=200)
+    doms = [_gpu(s) for s in scns]
+    b = bfvm.Batch(doms)
+    steps = 5
+    b.step(steps, graph_steps=2)
+    for k, (p, field, dirs, bulk, ag, internalized) in enumerate(scns):
+        f = field.astype(np.float32).astype(np.float64)
+        s = orc.Solver(p, f, dirichlet=dirs, bulk=list(bulk) if bulk is not None else None, agents=ag,
+                       compute_internalized=internalized)
+        s.solve(steps)
+        assert rel_linf(doms[k].field(), f).max() < 2e-6, k
+
+
+def test_domain_calls_after_batch_are_ordered():
+    require_gpu()
+    scns = _make(4, 5, extras=False)
+    doms = [_gpu(s) for s in scns]
+    ref = [_gpu(s) for s in scns]
+    b = bfvm.Batch(doms)
+    b.step(3)  # no sync: the domain streams wait for the batch by events
+    for d in doms:
+        d.step(2)
+    b.step(1)
+    for d in ref:
+        d.step(6)
+    for k, (d, r) in enumerate(zip(doms, ref)):
+        assert np.array_equal(d.field(), r.field()), k
+
+
+def test_batch_follows_agent_changes_and_releases():
+    require_gpu()
+    scns = [s for s in _make(5, 8, extras=True) if s[4] is not None][:4]
+    alone = [_gpu(s) for s in scns]
+    mixed = [_gpu(s) for s in scns]
+    b = bfvm.Batch(mixed)
+    b.step(2)
+    for d in alone:
+        d.step(2)
+    rng = np.random.default_rng(99)
+    for s, a, m in zip(scns, alone, mixed):
+        ag = s[4]
+        ag.positions[:] = np.clip(ag.positions + rng.normal(0, 5, ag.positions.shape), 0,
+                                  np.nextafter(np.array([s[0].nx, s[0].ny, s[0].nz][:s[0].dims]) * 20.0, 0))
+        for d in (a, m):
+            d.set_agents(ag.positions, ag.secretion, ag.saturation, ag.uptake, ag.net_export, None,
+                         ag.fraction_released, ag.volumes, ag.is_active)
+            d.release_internalized(0)
+    b.step(3, graph_steps=2)
+    for d in alone:
+        d.step(3)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a.field(), m.field()), k
+
+
+def test_batch_launch_grouping():
+    require_gpu()
+    p = orc.Problem(3, (16, 16, 16), (20, 20, 20), 0.01, [100.0, 10.0], [0.1, 0.2])
+    doms = [gpu_domain(p, field=np.ones(p.shape)) for _ in range(6)]
+    info = bfvm.Batch(doms).info()
+    # six identical domains share one kernel per axis
+    assert info["sweep_launches"] == 3
+    assert info["stage_launches"] == 0
+
+
+def test_batch_autotune_keeps_results():
+    require_gpu()
+    scns = _make(6, 6, extras=False, sides=[16, 24, 32])
+    doms = [_gpu(s) for s in scns]
+    ref = [_gpu(s) for s in scns]
+    b = bfvm.Batch(doms)
+    t = b.autotune(steps=2)
+    assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "no_pipeline"}
+    # the autotuner advanced the domains by 6 candidates x (2 + 2) steps
+    for d in ref:
+        d.step(24)
+    for k, (d, r) in enumerate(zip(doms, ref)):
+        assert rel_linf(d.field(), r.field()).max() < 1e-5, k
+
+
+def test_batch_rejects_bad_lists():
+    require_gpu()
+    p = orc.Problem(1, (8, 1, 1), (20, 20, 20), 0.01, [1.0], [0.1])
+    d = gpu_domain(p)
+    with pytest.raises(bfvm.BfvmError):
+        bfvm.Batch([])
+    with pytest.raises(bfvm.BfvmError):
+        bfvm.Batch([d, d])
