@@ -105,6 +105,98 @@ def cpu_baseline(w, budget_s=12.0):
                       f"cells, Dirichlet) in {el:.1f}s, after 1 warm-up step"}
 
 
+def build_c3_domains(seed=44, count=64, sides=tuple(range(32, 161, 16)), max_S=8, max_cells=50_000):
+    """Batched ensemble: `count` domains, each axis side drawn from `sides`
+    (anisotropic), dx=20, S uniform in 1..max_S with D/decay as the single large
+    domain, field U[0,1], 0..max_cells uniform cells (secretion/uptake U[0,10],
+    saturation 1, volume 2494).  Fields are generated and uploaded one domain at
+    a time."""
+    from physicore_amd import bfvm
+
+    rng = np.random.default_rng(seed)
+    doms, units = [], 0
+    for _ in range(count):
+        n = [int(rng.choice(sides)) for _ in range(3)]
+        S = int(rng.integers(1, max_S + 1))
+        D = 10.0 ** rng.uniform(0.0, 5.0, S)
+        lam = rng.uniform(0.01, 1.0, S)
+        d = bfvm.Domain(3, tuple(n), (20, 20, 20), 0.01, D, lam, np.zeros(S))
+        d.upload_field(rng.uniform(0.0, 1.0, (n[2], n[1], n[0], S)).astype(np.float32))
+        k = int(rng.integers(0, max_cells + 1))
+        if k:
+            box = np.array(n, dtype=np.float64) * 20.0
+            pos = np.minimum(rng.uniform(0.0, 1.0, (k, 3)) * box, np.nextafter(box, 0))
+            d.set_agents(pos, rng.uniform(0, 10, (k, S)), np.ones((k, S)), rng.uniform(0, 10, (k, S)),
+                         np.zeros((k, S)), None, np.zeros((k, S)), np.full(k, 2494.0), np.ones(k, dtype=np.uint8))
+        doms.append(d)
+        units += n[0] * n[1] * n[2] * S
+    return doms, units
+
+
+def bench_batched(steps, graph_steps=10, seed=44):
+    """The 64-domain ensemble stepped as one batch (after autotuning) and, for
+    comparison, the same domains stepped one after another on the plain path."""
+    from physicore_amd import bfvm
+
+    doms, units = build_c3_domains(seed)
+    b = bfvm.Batch(doms)
+    tuning = b.autotune(steps=3)
+    b.step(graph_steps, graph_steps)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps, graph_steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    info = b.info()
+    batched = units * steps / el
+    # plain: each domain alone (default tuning), one after the other
+    for d in doms:
+        d.set_tuning()
+        d.step_graph(graph_steps, graph_steps)
+        d.sync()
+    t0 = time.perf_counter()
+    for d in doms:
+        d.step_graph(steps, graph_steps)
+        d.sync()
+    el_plain = time.perf_counter() - t0
+    plain = units * steps / el_plain
+    gbs = units * 24 / (el / steps) / 1e9
+    b.close()
+    return {"workload": f"{len(doms)} domains, sides from {{32..160}} step 16 per axis, S 1-8, "
+                        f"0-50k cells each, seed {seed}",
+            "voxel_substrates": units, "steps": steps, "graph_steps": graph_steps,
+            "value": batched, "unit": "voxel*substrate updates/s",
+            "ms_per_step": el * 1e3 / steps,
+            "plain_value": plain, "plain_ms_per_step": el_plain * 1e3 / steps,
+            "batched_over_plain": batched / plain,
+            "launches_per_step": info, "tuning": tuning,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_step": units * 24}}
+
+
+def bench_served(count, rate, seed=46):
+    """Served stream: Poisson arrivals of requests (cube side log-uniform in
+    [32, 256], S 1-6, 0-200k cells, 100-5000 steps), HBM budget 200 GB,
+    continuous batching in chunks of 10 steps."""
+    from physicore_amd import serve
+
+    rng = np.random.default_rng(seed)
+    reqs = serve.poisson_stream(rng, count, rate, random_field=False)
+    srv = serve.Server(hbm_budget=200e9, chunk=10)
+    for r in reqs:
+        srv.submit(r)
+    t0 = srv.now()
+    done = srv.run()
+    wall = srv.now() - t0
+    ttc = [r.finished - r.arrival for r in done]
+    return {"workload": f"{count} requests, Poisson rate {rate}/s, side log-uniform [32,256], S 1-6, "
+                        f"0-200k cells, 100-5000 steps, budget 200 GB, chunk 10, seed {seed}",
+            "requests": len(done), "wall_s": wall,
+            "value": srv.updates / wall, "unit": "voxel*substrate updates/s",
+            "p50_s": serve.percentile(ttc, 50), "p99_s": serve.percentile(ttc, 99),
+            "chunks": srv.chunks}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +209,10 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--served-requests", type=int, default=32)
+    ap.add_argument("--served-rate", type=float, default=8.0)
+    ap.add_argument("--only-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
@@ -178,7 +274,14 @@ def main():
                           "diffusion_frac": units * 24 / (diff_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
         "stages_us": stages,
     }
+    del d
+    if not args.only_plain:
+        print("bench.py: batched ensemble", file=sys.stderr, flush=True)
+        out["batched"] = bench_batched(args.batched_steps)
+        print("bench.py: served stream", file=sys.stderr, flush=True)
+        out["served"] = bench_served(args.served_requests, args.served_rate)
     if not args.no_cpu_baseline:
+        print("bench.py: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
 

@@ -38,8 +38,8 @@ def run(cmd, cwd=None):
 
 
 HIP_SOURCES = ([f"bfvm_pipe_x{s}.hip" for s in (1, 2, 4, 8)] + [f"bfvm_pipe_{a}{g}.hip" for a in "yz" for g in range(6)] +
-               ["bfvm_sweeps_x64.hip", "bfvm_sweeps_y64.hip", "bfvm_sweeps_z64.hip",
-                "bfvm_sweeps_x32.hip", "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",
+               [f"bfvm_sweeps_{a}64{h}.hip" for a in "xyz" for h in "ab"] +
+               ["bfvm_sweeps_x32.hip", "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",
                 "bfvm_sweeps_rows.hip", "bfvm_sweeps_xscan.hip", "bfvm_sweeps.hip", "bfvm_kernels.hip",
                 "bfvm_context.hip"])
 

@@ -16,10 +16,25 @@
 #include "bfvm_internal.h"
 
 namespace {
-// x kernel of BFVM_X_AUTO: strided register lines (on 200^3 x 4, FP64
-// recurrences: 77 us, against 81 us for the scan with 2 workgroups per CU and
-// 99 us for the LDS transpose)
-constexpr int kDefaultXSweep = BFVM_X_STRIDED;
+// x kernel of BFVM_X_AUTO, from x-sweep timings on MI355X (FP64 recurrences,
+// nx x 128 x 128 domains, nx 32..200): with few substrates a lane's strided
+// line touches a new cache line at every step, so the row-parallel scan wins
+// (S = 1: 20 us against 63 us strided at nx = 200); the LDS transpose wins
+// for short rows of 2 substrates; from S = 4 the strided lines win
+// (200^3 x 4: 77 us, against 81 us for the scan and 99 us for the transpose).
+// shortest y/z line the pipelined kernel is used for by default
+constexpr int kPipeMinLine = 192;
+
+int default_x_sweep(int nx, int S)
+{
+	if (S == 1)
+		return BFVM_X_SCAN;
+	if (S == 2)
+		return nx >= 96 ? BFVM_X_SCAN : BFVM_X_TRANSPOSE;
+	if (S == 3)
+		return nx >= 128 ? BFVM_X_SCAN : BFVM_X_STRIDED;
+	return BFVM_X_STRIDED;
+}
 } // namespace
 
 using namespace bfvm;
@@ -158,7 +173,8 @@ struct bfvm_ctx
 	int64_t bulk_count = 0;
 	dvec<int64_t> bulk_vox;
 	dvec<double> bulk_a, bulk_b;
-	dvec<SparseRef> bulk_refs;
+	dvec<TileRef> bulk_tiles; // sparse tiles of the bulk list
+	int n_bulk_tiles = 0;
 
 	int64_t n_agents = 0;
 	dvec<double> positions, secretion, saturation, uptake, net_export, internalized_d, fraction_released, volumes;
@@ -167,7 +183,8 @@ struct bfvm_ctx
 	dvec<int32_t> cell_voxel, owner_of, ballot, owners, flags;
 	dvec<double> num, den, fac, rnum, rden, rfac, dnew, pnum, pden, pfac;
 	dvec<int32_t> pvox, slot_of;
-	dvec<SparseRef> agent_refs; // (0, k) for k < n_agents
+	dvec<TileRef> cell_tiles, int_tiles; // sparse tiles of the owner list / of all agents
+	int n_cell_tiles = 0, n_int_tiles = 0;
 	dvec<uint32_t> keys, vals, tkeys, tvals;
 	dvec<int32_t> hist, scan_tmp, heads, head_scan;
 	bool recompute_pending = false;
@@ -236,6 +253,15 @@ struct bfvm_ctx
 	} while (0)
 
 namespace {
+
+// Tiles (dom, first, count <= kSparseTile) over `count` entries of one domain.
+std::vector<TileRef> sparse_tiles(int dom, int64_t count)
+{
+	std::vector<TileRef> t;
+	for (int64_t k = 0; k < count; k += kSparseTile)
+		t.push_back({ dom, (int32_t)k, (int32_t)std::min<int64_t>(kSparseTile, count - k), 0 });
+	return t;
+}
 
 int sync_domain(bfvm_ctx* c)
 {
@@ -368,18 +394,19 @@ int build_tiles(bfvm_ctx* c)
 			const int n = (int)c->n[axis];
 			const int bucket = reg_bucket(n);
 			const int vpl = xscan_vpl(n, c->S);
-			const int xs = c->tuning.x_sweep == BFVM_X_AUTO ? kDefaultXSweep : c->tuning.x_sweep;
+			const int xs = c->tuning.x_sweep == BFVM_X_AUTO ? default_x_sweep(n, c->S) : c->tuning.x_sweep;
 			const int64_t lines = axis == 0 ? c->n[1] * c->n[2] * c->S : c->V / c->n[axis] * c->S;
 			const int pbucket = pipe_bucket(axis, n);
 			const PipeGeometry pg = pipe_geometry(axis, n, pbucket, c->S, c->L, lines);
 			c->pipe[axis] = PipeGeometry {};
 			if (c->tuning.force_lds_sweeps)
 				;
-			// y/z: pipelined register lines (FP64) wherever the shape fits; x
-			// only on request (its strided stores make it slower than the
-			// strided register kernel)
+			// y/z: pipelined register lines (FP64) for long lines (shorter ones
+			// gain nothing from it on MI355X: their tiles are too short to
+			// hide the next tile's load); x only on request (its strided
+			// stores make it slower than the strided register kernel)
 			else if (c->fp64 && !c->tuning.no_pipeline && pbucket && pg.waves > 0 &&
-					 (axis != 0 || c->tuning.x_sweep == BFVM_X_PIPELINED))
+					 (axis == 0 ? c->tuning.x_sweep == BFVM_X_PIPELINED : n >= kPipeMinLine))
 			{
 				c->variant[axis] = kSweepPipe;
 				c->nmax[axis] = pbucket;
@@ -433,9 +460,11 @@ int sweep_grid(const bfvm_ctx* c, int axis)
 	// factors in registers across rows; the others take one workgroup per tile
 	if (c->variant[axis] == kSweepPipe)
 	{
-		// persistent: one workgroup per CU (the LDS holds one), each looping
+		// persistent: as many workgroups per CU as the LDS holds (short lines
+		// leave room for several), each looping over rounds of tiles
 		const int rounds = c->ntiles[axis] / c->pipe[axis].waves;
-		return std::min(rounds, c->num_cu * std::max(per_cu, 1));
+		const int fit = std::max(1, (int)((size_t)kMaxLdsBytes / c->pipe[axis].lds_bytes));
+		return std::min(rounds, c->num_cu * (per_cu > 0 ? per_cu : fit));
 	}
 	if (per_cu == 0)
 		per_cu = c->variant[axis] == kSweepScan ? 2 : -1;
@@ -468,13 +497,13 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 		TRY_HIP(c, launch_dirichlet_interior(c->ddom.p, c->dir_refs.p, c->dir_count, c->stream));
 	TRY_HIP(c, mark(4));
 	if (c->bulk_count)
-		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_refs.p, c->bulk_count, c->S, c->stream));
+		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_tiles.p, c->n_bulk_tiles, c->stream));
 	TRY_HIP(c, mark(5));
 	if (c->n_owners > 0)
 	{
-		TRY_HIP(c, launch_cells_apply(c->ddom.p, c->agent_refs.p, c->n_owners, c->S, c->stream));
+		TRY_HIP(c, launch_cells_apply(c->ddom.p, c->cell_tiles.p, c->n_cell_tiles, c->stream));
 		if (c->internalized && c->conflict)
-			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->agent_refs.p, c->n_agents, c->S, c->stream));
+			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->int_tiles.p, c->n_int_tiles, c->stream));
 	}
 	TRY_HIP(c, mark(6));
 	return BFVM_OK;
@@ -546,7 +575,9 @@ int recompute(bfvm_ctx* c)
 	if (c->n_owners > 0)
 	{
 		TRY_HIP(c, launch_cells_pack(c->ddom.p, c->n_owners, s));
-		TRY_HIP(c, hipStreamSynchronize(s));
+		auto t = sparse_tiles(0, c->n_owners);
+		c->n_cell_tiles = (int)t.size();
+		TRY_HIP(c, c->cell_tiles.upload(t.data(), t.size(), s)); // synchronizes s
 	}
 	return BFVM_OK;
 }
@@ -665,6 +696,11 @@ struct bfvm_batch
 	hipStream_t stream = nullptr;
 	int num_cu = 256;
 	hipEvent_t ev_in = nullptr, ev_out = nullptr;
+	// the groups of one axis touch disjoint domains: they run on kBatchLanes
+	// streams so that the tail of one launch overlaps the next
+	static constexpr int kBatchLanes = 4;
+	hipStream_t lane[kBatchLanes - 1] = {};
+	hipEvent_t ev_fork = nullptr, ev_join[kBatchLanes - 1] = {};
 
 	struct SweepLaunch
 	{
@@ -678,13 +714,17 @@ struct bfvm_batch
 	};
 	struct RefLaunch
 	{
-		int S = 0;
 		int64_t n = 0;
 		dvec<SparseRef> refs;
 	};
+	struct TileLaunch
+	{
+		int n = 0;
+		dvec<TileRef> tiles;
+	};
 	std::vector<std::unique_ptr<SweepLaunch>> sweeps;
 	std::unique_ptr<RefLaunch> dirichlet;
-	std::vector<std::unique_ptr<RefLaunch>> bulk, cells, internalized;
+	std::unique_ptr<TileLaunch> bulk, cells, internalized;
 	dvec<DomainDev> ddoms;
 	std::vector<uint64_t> signature;
 
@@ -720,6 +760,15 @@ struct bfvm_batch
 			(void)hipEventDestroy(ev_in);
 		if (ev_out)
 			(void)hipEventDestroy(ev_out);
+		if (ev_fork)
+			(void)hipEventDestroy(ev_fork);
+		for (int k = 0; k < kBatchLanes - 1; k++)
+		{
+			if (ev_join[k])
+				(void)hipEventDestroy(ev_join[k]);
+			if (lane[k])
+				(void)hipStreamDestroy(lane[k]);
+		}
 		if (stream)
 			(void)hipStreamDestroy(stream);
 	}
@@ -838,7 +887,8 @@ int batch_prepare(bfvm_batch* b)
 					domain_pipe_tiles(axis, i, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, g, t);
 				}
 				L->ntiles = (int)t.size();
-				L->grid = std::min(L->ntiles / g.waves, b->num_cu);
+				const int fit = std::max(1, (int)((size_t)kMaxLdsBytes / g.lds_bytes));
+				L->grid = std::min(L->ntiles / g.waves, b->num_cu * fit);
 			}
 			else
 			{
@@ -860,73 +910,110 @@ int batch_prepare(bfvm_batch* b)
 		}
 	}
 
-	// boundary, bulk and cells: refs (domain, entry), per substrate count
-	auto make_refs = [&](auto count_of, std::vector<std::unique_ptr<bfvm_batch::RefLaunch>>& out,
-						 bool by_S) -> int {
-		out.clear();
-		std::map<int, std::vector<SparseRef>> by;
+	// boundary: refs (domain, entry); bulk and cells: sparse tiles of every
+	// domain, one launch each
+	{
+		std::vector<SparseRef> v;
+		for (size_t i = 0; i < b->ctx.size(); i++)
+			for (int64_t k = 0; k < b->ctx[i]->dir_count; k++)
+				v.push_back({ (int32_t)i, (int32_t)k });
+		b->dirichlet.reset();
+		if (!v.empty())
+		{
+			b->dirichlet = std::make_unique<bfvm_batch::RefLaunch>();
+			b->dirichlet->n = (int64_t)v.size();
+			if (hipError_t e = b->dirichlet->refs.upload(v.data(), v.size(), b->stream); e != hipSuccess)
+				return b->hip_fail(e, "ref upload");
+		}
+	}
+	auto make_tiles = [&](auto count_of, std::unique_ptr<bfvm_batch::TileLaunch>& out) -> int {
+		std::vector<TileRef> t;
 		for (size_t i = 0; i < b->ctx.size(); i++)
 		{
-			const int64_t n = count_of(b->ctx[i]);
-			auto& v = by[by_S ? b->ctx[i]->S : 0];
-			for (int64_t k = 0; k < n; k++)
-				v.push_back({ (int32_t)i, (int32_t)k });
+			auto ti = sparse_tiles((int)i, count_of(b->ctx[i]));
+			t.insert(t.end(), ti.begin(), ti.end());
 		}
-		for (auto& [S, v] : by)
-		{
-			if (v.empty())
-				continue;
-			auto R = std::make_unique<bfvm_batch::RefLaunch>();
-			R->S = S;
-			R->n = (int64_t)v.size();
-			if (hipError_t e = R->refs.upload(v.data(), v.size(), b->stream); e != hipSuccess)
-				return b->hip_fail(e, "ref upload");
-			out.push_back(std::move(R));
-		}
+		out.reset();
+		if (t.empty())
+			return BFVM_OK;
+		out = std::make_unique<bfvm_batch::TileLaunch>();
+		out->n = (int)t.size();
+		if (hipError_t e = out->tiles.upload(t.data(), t.size(), b->stream); e != hipSuccess)
+			return b->hip_fail(e, "tile upload");
 		return BFVM_OK;
 	};
-	std::vector<std::unique_ptr<bfvm_batch::RefLaunch>> dir;
-	int rc = make_refs([](const bfvm_ctx* c) { return c->dir_count; }, dir, false);
+	int rc = make_tiles([](const bfvm_ctx* c) { return c->bulk_count; }, b->bulk);
 	if (!rc)
-		rc = make_refs([](const bfvm_ctx* c) { return c->bulk_count; }, b->bulk, true);
+		rc = make_tiles([](const bfvm_ctx* c) { return c->n_owners; }, b->cells);
 	if (!rc)
-		rc = make_refs([](const bfvm_ctx* c) { return c->n_owners; }, b->cells, true);
-	if (!rc)
-		rc = make_refs([](const bfvm_ctx* c) { return c->internalized && c->conflict ? c->n_agents : 0; },
-					   b->internalized, true);
+		rc = make_tiles([](const bfvm_ctx* c) { return c->internalized && c->conflict ? c->n_agents : 0; },
+						b->internalized);
 	if (rc)
 		return rc;
-	b->dirichlet = dir.empty() ? nullptr : std::move(dir[0]);
 	b->signature = sig;
 	return BFVM_OK;
 }
 
 int batch_enqueue_step(bfvm_batch* b)
 {
-	for (auto& L : b->sweeps)
+	constexpr int K = bfvm_batch::kBatchLanes;
+	for (int axis = 0; axis < 3; axis++)
 	{
-		hipError_t e;
-		if (L->variant == kSweepPipe)
-			e = launch_sweep_pipe(L->axis, L->nmax, L->faces, b->ddoms.p, L->tiles.p, L->ntiles, 1, L->fac_cap, L->pg,
-								  L->grid, b->stream);
-		else
-			e = launch_sweep(L->axis, L->variant, L->nmax, L->fp64, L->faces, L->S, b->ddoms.p, L->tiles.p,
-							 L->ntiles, 1, L->fac_cap, L->lds, L->grid, b->stream);
-		if (e != hipSuccess)
-			return b->hip_fail(e, "sweep launch");
+		std::vector<bfvm_batch::SweepLaunch*> g;
+		for (auto& L : b->sweeps)
+			if (L->axis == axis)
+				g.push_back(L.get());
+		if (g.empty())
+			continue;
+		// biggest launches first, dealt round-robin over the streams
+		std::stable_sort(g.begin(), g.end(), [](auto* a, auto* c) { return a->ntiles > c->ntiles; });
+		const int lanes = std::min<int>(K, (int)g.size());
+		if (lanes > 1)
+		{
+			if (hipError_t e = hipEventRecord(b->ev_fork, b->stream); e != hipSuccess)
+				return b->hip_fail(e, "hipEventRecord");
+			for (int k = 0; k < lanes - 1; k++)
+				if (hipError_t e = hipStreamWaitEvent(b->lane[k], b->ev_fork, 0); e != hipSuccess)
+					return b->hip_fail(e, "hipStreamWaitEvent");
+		}
+		for (size_t j = 0; j < g.size(); j++)
+		{
+			bfvm_batch::SweepLaunch* L = g[j];
+			const int k = (int)(j % lanes);
+			hipStream_t st = k == 0 ? b->stream : b->lane[k - 1];
+			hipError_t e;
+			if (L->variant == kSweepPipe)
+				e = launch_sweep_pipe(L->axis, L->nmax, L->faces, b->ddoms.p, L->tiles.p, L->ntiles, 1, L->fac_cap,
+									  L->pg, L->grid, st);
+			else
+				e = launch_sweep(L->axis, L->variant, L->nmax, L->fp64, L->faces, L->S, b->ddoms.p, L->tiles.p,
+								 L->ntiles, 1, L->fac_cap, L->lds, L->grid, st);
+			if (e != hipSuccess)
+				return b->hip_fail(e, "sweep launch");
+		}
+		// the next axis starts when every group of this one is done
+		for (int k = 0; k < lanes - 1; k++)
+		{
+			if (hipError_t e = hipEventRecord(b->ev_join[k], b->lane[k]); e != hipSuccess)
+				return b->hip_fail(e, "hipEventRecord");
+			if (hipError_t e = hipStreamWaitEvent(b->stream, b->ev_join[k], 0); e != hipSuccess)
+				return b->hip_fail(e, "hipStreamWaitEvent");
+		}
 	}
 	if (b->dirichlet)
 		if (hipError_t e = launch_dirichlet_interior(b->ddoms.p, b->dirichlet->refs.p, b->dirichlet->n, b->stream);
 			e != hipSuccess)
 			return b->hip_fail(e, "dirichlet launch");
-	for (auto& R : b->bulk)
-		if (hipError_t e = launch_bulk(b->ddoms.p, R->refs.p, R->n, R->S, b->stream); e != hipSuccess)
+	if (b->bulk)
+		if (hipError_t e = launch_bulk(b->ddoms.p, b->bulk->tiles.p, b->bulk->n, b->stream); e != hipSuccess)
 			return b->hip_fail(e, "bulk launch");
-	for (auto& R : b->cells)
-		if (hipError_t e = launch_cells_apply(b->ddoms.p, R->refs.p, R->n, R->S, b->stream); e != hipSuccess)
+	if (b->cells)
+		if (hipError_t e = launch_cells_apply(b->ddoms.p, b->cells->tiles.p, b->cells->n, b->stream); e != hipSuccess)
 			return b->hip_fail(e, "cells launch");
-	for (auto& R : b->internalized)
-		if (hipError_t e = launch_cells_internalized(b->ddoms.p, R->refs.p, R->n, R->S, b->stream); e != hipSuccess)
+	if (b->internalized)
+		if (hipError_t e = launch_cells_internalized(b->ddoms.p, b->internalized->tiles.p, b->internalized->n,
+													 b->stream);
+			e != hipSuccess)
 			return b->hip_fail(e, "internalized launch");
 	return BFVM_OK;
 }
@@ -1227,8 +1314,9 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 	TRY_HIP(c, c->bulk_vox.upload(vox.data(), vox.size(), c->stream));
 	TRY_HIP(c, c->bulk_a.upload(a.data(), a.size(), c->stream));
 	TRY_HIP(c, c->bulk_b.upload(b.data(), b.size(), c->stream));
-	auto r = refs_for(count);
-	TRY_HIP(c, c->bulk_refs.upload(r.data(), r.size(), c->stream));
+	auto t = sparse_tiles(0, count);
+	c->n_bulk_tiles = (int)t.size();
+	TRY_HIP(c, c->bulk_tiles.upload(t.data(), t.size(), c->stream));
 	return sync_domain(c);
 }
 
@@ -1278,8 +1366,9 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->pfac.alloc(nS));
 	TRY_HIP(c, c->pvox.alloc(n));
 	TRY_HIP(c, c->slot_of.alloc(n));
-	auto r = refs_for(n);
-	TRY_HIP(c, c->agent_refs.upload(r.data(), r.size(), c->stream));
+	auto t = sparse_tiles(0, n);
+	c->n_int_tiles = (int)t.size();
+	TRY_HIP(c, c->int_tiles.upload(t.data(), t.size(), c->stream));
 	c->recompute_pending = true;
 	c->n_owners = 0;
 	return sync_domain(c);
@@ -1626,6 +1715,14 @@ int bfvm_batch_create(bfvm_ctx* const* ctxs, int32_t count, bfvm_batch** out)
 		e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
 	if (e == hipSuccess)
 		e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+	if (e == hipSuccess)
+		e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+	for (int k = 0; k < bfvm_batch::kBatchLanes - 1 && e == hipSuccess; k++)
+	{
+		e = hipStreamCreateWithFlags(&b->lane[k], hipStreamNonBlocking);
+		if (e == hipSuccess)
+			e = hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming);
+	}
 	if (e != hipSuccess)
 	{
 		g_error = std::string("batch stream/events: ") + hipGetErrorString(e);
@@ -1672,7 +1769,8 @@ int bfvm_batch_info(bfvm_batch* b, int32_t* sweep_launches, int32_t* stage_launc
 	if (sweep_launches)
 		*sweep_launches = (int32_t)b->sweeps.size();
 	if (stage_launches)
-		*stage_launches = (int32_t)((b->dirichlet ? 1 : 0) + b->bulk.size() + b->cells.size() + b->internalized.size());
+		*stage_launches = (int32_t)((b->dirichlet ? 1 : 0) + (b->bulk ? 1 : 0) + (b->cells ? 1 : 0) +
+									(b->internalized ? 1 : 0));
 	return BFVM_OK;
 }
 

@@ -93,7 +93,7 @@ struct SparseRef
 };
 
 // Sweep kernel variants: lines held in VGPRs (n <= kRegMaxN, templated on a
-// bucket of 32) or solved inside an LDS tile (any n the LDS can hold).
+// bucket of 16) or solved inside an LDS tile (any n the LDS can hold).
 enum SweepVariant
 {
 	kSweepLds = 0,
@@ -130,7 +130,7 @@ __host__ inline int xscan_vpl(int nx, int S)
 }
 constexpr int kRegMaxN = 256;
 constexpr int kColsPerTile = kSweepThreads; // y/z register tiles: 256 flattened columns
-int reg_bucket(int n);						// 32..256, or 0 when n > kRegMaxN
+int reg_bucket(int n);						// 16..256, or 0 when n > kRegMaxN
 
 // LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);
@@ -174,11 +174,11 @@ hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, 
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
 // The per-step sparse stages run one thread per (entry, substrate); `S` is the
 // substrate count of every referenced domain (batched launches group by S).
-hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, int S, hipStream_t stream);
-hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* owner_refs, int64_t n, int S,
-							  hipStream_t stream);
-hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* cell_refs, int64_t n, int S,
-									 hipStream_t stream);
+// Sparse stages over tiles (dom, first entry, count <= kSparseTile).
+constexpr int kSparseTile = 256;
+hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
+hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
+hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
 hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);
 hipError_t launch_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream);

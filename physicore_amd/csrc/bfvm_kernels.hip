@@ -35,18 +35,21 @@ __global__ void dirichlet_interior_kernel(const DomainDev* __restrict__ doms, co
 // Bulk supply/uptake (bulk_solver.cpp:12-41): D = (D + dt*S*T) / (1 + dt*(U+S)),
 // with a = dt*S*T and b = 1 + dt*(U+S) precomputed in FP64 on the host.  One
 // thread per (listed voxel, substrate).
-__global__ void bulk_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs, int64_t n, int S)
+// Sparse stages (bulk, cell apply, per-agent internalized) run over tiles of
+// up to kSparseTile entries of one domain: tile = (dom, first entry, count), one
+// workgroup per tile, so the domain's descriptor is read once into SGPRs.
+__global__ void bulk_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
-	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n * S)
-		return;
-	const int64_t k = t / S;
-	const int s = (int)(t - k * S);
-	const SparseRef r = refs[k];
-	const DomainDev& d = doms[r.dom];
-	const int64_t q = (int64_t)r.idx * S + s;
-	float* D = d.field + d.bulk_vox[r.idx] * S + s;
-	*D = (float)(((double)*D + d.bulk_a[q]) / d.bulk_b[q]);
+	const TileRef tr = tiles[blockIdx.x];
+	const DomainDev& d = doms[tr.dom];
+	const int S = d.S;
+	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+	{
+		const int k = tr.a + t / S, s = t % S;
+		const int64_t q = (int64_t)k * S + s;
+		float* D = d.field + d.bulk_vox[k] * S + s;
+		*D = (float)(((double)*D + d.bulk_a[q]) / d.bulk_b[q]);
+	}
 }
 
 // ---------------------------------------------------------------- cells
@@ -170,14 +173,24 @@ __global__ __launch_bounds__(256) void cells_ballot_kernel(const DomainDev* __re
 	}
 }
 
+// Appends the owners to the owner list: one atomic per wave on the shared
+// counter (the leader reserves the wave's slots, lanes take their rank).
 __global__ void cells_collect_owners_kernel(const DomainDev* __restrict__ dp, int64_t n)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
 	const DomainDev& d = *dp;
-	if (d.cell_voxel[i] >= 0 && d.owner_of[i] == (int)i)
-		d.owners[atomicAdd(&d.flags[2], 1)] = (int)i;
+	const bool own = i < n && d.cell_voxel[i] >= 0 && d.owner_of[i] == (int)i;
+	const uint64_t m = __ballot(own);
+	if (m == 0)
+		return;
+	const int leader = __ffsll((unsigned long long)m) - 1;
+	int base = 0;
+	if ((int)__lane_id() == leader)
+		base = atomicAdd(&d.flags[2], __popcll(m));
+	base = __shfl(base, leader);
+	const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+	if (own)
+		d.owners[base + rank] = (int)i;
 }
 
 __global__ void cells_make_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals, int64_t n)
@@ -272,45 +285,43 @@ __global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_ow
 // per-agent update.  One thread per (occupied voxel, substrate).  Agent
 // activity is fixed between recomputes (new agent arrays always trigger one),
 // so every listed owner is active.
-__global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs, int64_t n,
-								   int S)
+__global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
-	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n * S)
-		return;
-	const int64_t kk = t / S;
-	const int s = (int)(t - kk * S);
-	const SparseRef r = refs[kk];
-	const DomainDev& d = doms[r.dom];
-	const int64_t q = (int64_t)r.idx * S + s;
-	float* D = d.field + (int64_t)d.pvox[r.idx] * S + s;
-	const double d0 = (double)*D;
-	const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
-	*D = (float)d1;
-	if (d.compute_internalized && !d.flags[0])
-		d.internalized[(int64_t)d.owners[r.idx] * S + s] += d.voxel_volume * (d0 - d1);
-	else
-		d.dnew[q] = d1;
+	const TileRef tr = tiles[blockIdx.x];
+	const DomainDev& d = doms[tr.dom];
+	const int S = d.S;
+	const bool fused = d.compute_internalized && !d.flags[0];
+	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+	{
+		const int k = tr.a + t / S, s = t % S;
+		const int64_t q = (int64_t)k * S + s;
+		float* D = d.field + (int64_t)d.pvox[k] * S + s;
+		const double d0 = (double)*D;
+		const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
+		*D = (float)d1;
+		if (fused)
+			d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
+		else
+			d.dnew[q] = d1;
+	}
 }
 
 // compute_internalized (cell_solver.cpp:131-144, :229-243):
 // I -= V*(num - D_new*den + fac) for every active agent, when voxels are shared.
-__global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const SparseRef* __restrict__ refs,
-										  int64_t n, int S)
+__global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
-	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n * S)
-		return;
-	const int64_t kk = t / S;
-	const int s = (int)(t - kk * S);
-	const SparseRef r = refs[kk];
-	const DomainDev& d = doms[r.dom];
-	const int64_t i = r.idx;
-	if (d.cell_voxel[i] < 0)
-		return;
-	const int64_t q = i * S + s;
-	const int64_t slot = d.slot_of[d.owner_of[i]];
-	d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[slot * S + s] * d.den[q] + d.fac[q]);
+	const TileRef tr = tiles[blockIdx.x];
+	const DomainDev& d = doms[tr.dom];
+	const int S = d.S;
+	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+	{
+		const int i = tr.a + t / S, s = t % S;
+		if (d.cell_voxel[i] < 0)
+			continue;
+		const int64_t q = (int64_t)i * S + s;
+		const int64_t slot = d.slot_of[d.owner_of[i]];
+		d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[slot * S + s] * d.den[q] + d.fac[q]);
+	}
 }
 
 // release_internal (cell_solver.cpp:274-289): D += I*F/V, I = 0, for a batch
@@ -576,20 +587,28 @@ hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* ref
 	return launch1d(dirichlet_interior_kernel, n, 256, stream, doms, refs, n);
 }
 
-hipError_t launch_bulk(const DomainDev* doms, const SparseRef* refs, int64_t n, int S, hipStream_t stream)
+hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream)
 {
-	return launch1d(bulk_kernel, n * S, 256, stream, doms, refs, n, S);
+	if (ntiles <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(bulk_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
+	return hipGetLastError();
 }
 
-hipError_t launch_cells_apply(const DomainDev* doms, const SparseRef* refs, int64_t n, int S, hipStream_t stream)
+hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream)
 {
-	return launch1d(cells_apply_kernel, n * S, 256, stream, doms, refs, n, S);
+	if (ntiles <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(cells_apply_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
+	return hipGetLastError();
 }
 
-hipError_t launch_cells_internalized(const DomainDev* doms, const SparseRef* refs, int64_t n, int S,
-									 hipStream_t stream)
+hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream)
 {
-	return launch1d(cells_internalized_kernel, n * S, 256, stream, doms, refs, n, S);
+	if (ntiles <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(cells_internalized_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
+	return hipGetLastError();
 }
 
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream)

@@ -8,10 +8,13 @@
 
 namespace bfvm {
 namespace sweep {
-kernel_fn pick_cols_x64(int nmax, bool faces);
+kernel_fn pick_cols_x64a(int nmax, bool faces);
+kernel_fn pick_cols_x64b(int nmax, bool faces);
+kernel_fn pick_cols_y64a(int nmax, bool faces);
+kernel_fn pick_cols_y64b(int nmax, bool faces);
+kernel_fn pick_cols_z64a(int nmax, bool faces);
+kernel_fn pick_cols_z64b(int nmax, bool faces);
 kernel_fn pick_cols_x32(int nmax, bool faces);
-kernel_fn pick_cols_y64(int nmax, bool faces);
-kernel_fn pick_cols_z64(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
@@ -89,11 +92,13 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		return fp64 ? pick_lds<double>(axis) : pick_lds<float>(axis);
 	if (variant == kSweepRows)
 		return axis == 0 ? pick_rows(nmax, fp64) : nullptr;
+	// FP64 instantiations split at 128 lines (two translation units per axis)
+	const bool lo = nmax <= 128;
 	if (axis == 0)
-		return fp64 ? pick_cols_x64(nmax, faces) : pick_cols_x32(nmax, faces);
+		return fp64 ? (lo ? pick_cols_x64a(nmax, faces) : pick_cols_x64b(nmax, faces)) : pick_cols_x32(nmax, faces);
 	if (axis == 1)
-		return fp64 ? pick_cols_y64(nmax, faces) : pick_cols_y32(nmax, faces);
-	return fp64 ? pick_cols_z64(nmax, faces) : pick_cols_z32(nmax, faces);
+		return fp64 ? (lo ? pick_cols_y64a(nmax, faces) : pick_cols_y64b(nmax, faces)) : pick_cols_y32(nmax, faces);
+	return fp64 ? (lo ? pick_cols_z64a(nmax, faces) : pick_cols_z64b(nmax, faces)) : pick_cols_z32(nmax, faces);
 }
 } // namespace
 } // namespace sweep
@@ -102,7 +107,7 @@ int reg_bucket(int n)
 {
 	if (n > kRegMaxN)
 		return 0;
-	return ((n + 31) / 32) * 32;
+	return ((n + 15) / 16) * 16;
 }
 
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
@@ -121,7 +126,7 @@ hipError_t init_sweep_kernels()
 {
 	const int max_lds = kMaxLdsBytes;
 	for (int axis = 0; axis < 3; axis++)
-		for (int nmax = 0; nmax <= kRegMaxN; nmax += 32)
+		for (int nmax = 0; nmax <= kRegMaxN; nmax += 16)
 			for (int p = 0; p < 8; p++)
 			{
 				int variant = nmax == 0 ? kSweepLds : kSweepReg;

@@ -1,24 +1,25 @@
-// Instantiations of the x-sweep register kernel (strided lines), FP64 arithmetic.
+// Instantiations of the x-sweep register kernel, FP64 arithmetic,
+// lines of 1..128 voxels in buckets of 16.
 #include "bfvm_sweep_kernels.h"
 
 namespace bfvm {
 namespace sweep {
 
-kernel_fn pick_cols_x64(int nmax, bool faces)
+kernel_fn pick_cols_x64a(int nmax, bool faces)
 {
 #define BFVM_BUCKET(N) \
 	case N:            \
 		return faces ? sweep_cols_kernel<0, N, double, true> : sweep_cols_kernel<0, N, double, false>;
 	switch (nmax)
 	{
+		BFVM_BUCKET(16)
 		BFVM_BUCKET(32)
+		BFVM_BUCKET(48)
 		BFVM_BUCKET(64)
+		BFVM_BUCKET(80)
 		BFVM_BUCKET(96)
+		BFVM_BUCKET(112)
 		BFVM_BUCKET(128)
-		BFVM_BUCKET(160)
-		BFVM_BUCKET(192)
-		BFVM_BUCKET(224)
-		BFVM_BUCKET(256)
 	default:
 		return nullptr;
 	}

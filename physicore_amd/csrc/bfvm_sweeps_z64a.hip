@@ -1,0 +1,30 @@
+// Instantiations of the z-sweep register kernel, FP64 arithmetic,
+// lines of 1..128 voxels in buckets of 16.
+#include "bfvm_sweep_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_cols_z64a(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_cols_kernel<2, N, double, true> : sweep_cols_kernel<2, N, double, false>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(16)
+		BFVM_BUCKET(32)
+		BFVM_BUCKET(48)
+		BFVM_BUCKET(64)
+		BFVM_BUCKET(80)
+		BFVM_BUCKET(96)
+		BFVM_BUCKET(112)
+		BFVM_BUCKET(128)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

@@ -32,7 +32,8 @@ class Request:
     diffusion: np.ndarray
     decay: np.ndarray
     steps: int
-    field: Optional[np.ndarray] = None        # (nz, ny, nx, S) or None (zeros)
+    field: Optional[np.ndarray] = None        # (nz, ny, nx, S), or None: `initial`
+    initial: Optional[np.ndarray] = None      # per-substrate initial value (zeros if None)
     faces: Optional[tuple] = None             # (values[6, S], conds[6, S])
     bulk: Optional[tuple] = None              # (voxels, supply, uptake, target)
     agents: Optional[dict] = None             # keyword arguments of Domain.set_agents
@@ -99,7 +100,8 @@ class Server:
         still = []
         for r in self.waiting:
             if r.arrival <= t and self.in_use + r.device_bytes() <= self.budget:
-                d = bfvm.Domain(r.dims, r.grid, r.voxel, r.dt, r.diffusion, r.decay, reduce=r.reduce)
+                d = bfvm.Domain(r.dims, r.grid, r.voxel, r.dt, r.diffusion, r.decay, initial=r.initial,
+                                reduce=r.reduce)
                 if r.field is not None:
                     d.upload_field(r.field)
                 if r.faces is not None:
@@ -172,10 +174,11 @@ def percentile(values, q):
 
 
 def poisson_stream(rng, count, rate, side_range=(32, 256), s_range=(1, 6), max_cells=200_000,
-                   step_range=(100, 5000), dx=20, dt=0.01):
+                   step_range=(100, 5000), dx=20, dt=0.01, random_field=True):
     """Requests of the served workload: cube side log-uniform in side_range,
     S uniform in s_range, D log-uniform in [1, 1e5], decay uniform in
-    [0.01, 1], field U[0,1], Dirichlet 1.0 on x-min, 0..max_cells uniform cells
+    [0.01, 1], field U[0,1] (random_field) or a per-substrate constant drawn
+    U[0,1], Dirichlet 1.0 on x-min, 0..max_cells uniform cells
     (secretion/uptake U[0,10], saturation 1, volume 2494), steps log-uniform
     in step_range; exponential inter-arrival times at `rate` per second."""
     reqs = []
@@ -199,7 +202,9 @@ def poisson_stream(rng, count, rate, side_range=(32, 256), s_range=(1, 6), max_c
                           uptake=rng.uniform(0, 10, (n, S)), net_export=np.zeros((n, S)), internalized=None,
                           fraction_released=np.zeros((n, S)), volumes=np.full(n, 2494.0),
                           is_active=np.ones(n, dtype=np.uint8))
+        field = rng.uniform(0.0, 1.0, (side, side, side, S)) if random_field else None
+        initial = None if random_field else rng.uniform(0.0, 1.0, S)
         reqs.append(Request(dims=3, grid=(side, side, side), voxel=(dx, dx, dx), dt=dt, diffusion=D, decay=lam,
-                            steps=steps, field=rng.uniform(0.0, 1.0, (side, side, side, S)),
-                            faces=(vals, conds), agents=agents, arrival=t, rid=i))
+                            steps=steps, field=field, initial=initial, faces=(vals, conds), agents=agents,
+                            arrival=t, rid=i))
     return reqs

@@ -41,8 +41,10 @@ def test_results_same_alone_and_in_the_mix():
     assert any(a.admitted < b.finished and b.admitted < a.finished
                for a in done for b in done if a.rid < b.rid)
     for r in reqs:
-        d = bfvm.Domain(r.dims, r.grid, r.voxel, r.dt, r.diffusion, r.decay, reduce=r.reduce)
-        d.upload_field(r.field)
+        d = bfvm.Domain(r.dims, r.grid, r.voxel, r.dt, r.diffusion, r.decay, initial=r.initial,
+                        reduce=r.reduce)
+        if r.field is not None:
+            d.upload_field(r.field)
         d.set_dirichlet_faces(*r.faces)
         if r.agents is not None:
             d.set_agents(**r.agents)
