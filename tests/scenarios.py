@@ -232,3 +232,28 @@ def mixed_domain(rng, sides, max_S=8, max_cells=50_000, dims=3, extras=False, dx
             ag.fraction_released[:] = rng.uniform(0.0, 1.0, (ncell, S))
             ag.is_active[:] = rng.uniform(0, 1, ncell) < 0.9
     return p, field, dirs, bulk, ag, internalized
+
+
+def config4(seed=45, n_cells=2_000_000, nx=400, ny=400, nz=100):
+    """Dense agents: nx x ny x nz voxels at dx=10 (400x400x100: box 4000x4000x1000),
+    S=2 with D={1e5, 100}, decay {0.1, 0.01}, field U[0,1]; cells drawn from an
+    isotropic Gaussian (sigma 300um) at the box centre, redrawn until inside the
+    box; secretion/uptake U[0,10] (saturation 1, volume 2494), net export U[-1,1]."""
+    rng = np.random.default_rng(seed)
+    dx = 10
+    p = orc.Problem(3, (nx, ny, nz), (dx, dx, dx), 0.01, [1e5, 100.0], [0.1, 0.01])
+    field = rng.uniform(0.0, 1.0, p.shape)
+    box = np.array([nx * dx, ny * dx, nz * dx], dtype=np.float64)
+    pos = np.empty((0, 3))
+    while len(pos) < n_cells:
+        c = rng.normal(box / 2, 300.0, (n_cells, 3))
+        c = c[((c >= 0) & (c < box)).all(axis=1)]
+        pos = np.concatenate([pos, c])[:n_cells]
+    ag = orc.AgentSpec(n_cells, 2, 3)
+    ag.positions[:] = pos
+    ag.secretion[:] = rng.uniform(0.0, 10.0, (n_cells, 2))
+    ag.uptake[:] = rng.uniform(0.0, 10.0, (n_cells, 2))
+    ag.saturation[:] = 1.0
+    ag.volumes[:] = 2494.0
+    ag.net_export[:] = rng.uniform(-1.0, 1.0, (n_cells, 2))
+    return p, field, None, None, ag
