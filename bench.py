@@ -213,10 +213,24 @@ def main():
     ap.add_argument("--served-requests", type=int, default=32)
     ap.add_argument("--served-rate", type=float, default=8.0)
     ap.add_argument("--only-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MB device buffer 10 times (reference for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.rand(64 * 1024 * 1024, device="cuda:0")  # 256 MB of FP32
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                          "launches": 10}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
-              "(domains shard by HBM budget or z-slabs, described in DESIGN.md)", file=sys.stderr)
+              "(independent domains would spread over GPUs by HBM budget, one large domain by z-slabs)",
+              file=sys.stderr)
         sys.exit(2)
 
     from physicore_amd import bfvm

@@ -36,7 +36,14 @@ typedef enum bfvm_status
 	BFVM_ERR_STATE = -6		   /* call out of order (e.g. no agents set) */
 } bfvm_status;
 
-/* Arithmetic of the Thomas recurrences.  The field is FP32 in HBM in both. */
+/* Arithmetic of the Thomas recurrences.  The field is FP32 in HBM in both.
+ * With FP64 recurrences every FP64 result written to FP32 (sweep outputs,
+ * forward values stored in FP32, bulk and cell updates) is rounded
+ * stochastically, from random bits that are a function of the domain's step
+ * count, the stage and the position: the error has mean zero, so it does not
+ * pile up near slowly converging states as round-to-nearest does, and results
+ * are reproducible and independent of batching.  bfvm_tuning.round_nearest
+ * selects round-to-nearest instead. */
 typedef enum bfvm_precision
 {
 	BFVM_SWEEP_FP64 = 0, /* FP64 recurrences on FP32 storage (default) */
@@ -190,21 +197,23 @@ typedef enum bfvm_x_sweep
 						   * (FP64, S in {1, 2, 4, 8}, nx*S % 4 == 0) */
 } bfvm_x_sweep;
 
-/* Launch parameters of the sweep kernels.  persistent_wgs_per_cu: 0 = the
- * default per kernel, -1 = one workgroup per tile, > 0 = that many workgroups
- * per CU, each looping over tiles.  force_lds_sweeps = 1 solves every line
- * inside an LDS tile instead of in registers (the variant used for lines
- * longer than 256).  x_sweep: a bfvm_x_sweep value.  no_pipeline = 1 turns
- * off the pipelined register sweeps of y and z (FP64, lines up to 256,
- * nx*S % 4 == 0), in which the next tile of 64 lines streams into LDS while
- * the current one is solved. */
+/* Kernel selection and rounding.  persistent_wgs_per_cu: 0 = the default per
+ * kernel, -1 = one workgroup per tile, > 0 = that many workgroups per CU, each
+ * looping over tiles.  force_lds_sweeps = 1 solves every line inside an LDS
+ * tile instead of in registers (the variant used for lines longer than 256).
+ * x_sweep: a bfvm_x_sweep value.  pipeline = 1 solves y and z lines of
+ * 192..256 voxels (FP64, nx*S % 4 == 0) with the pipelined register sweep, in
+ * which the next tile of 64 lines streams into LDS while the current one is
+ * solved.  round_nearest = 1 rounds FP64 results to the FP32 field to nearest
+ * instead of stochastically (see bfvm_precision). */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
 	int32_t force_lds_sweeps;
 	int32_t x_sweep;
-	int32_t no_pipeline;
-	int32_t reserved[4];
+	int32_t pipeline;
+	int32_t round_nearest;
+	int32_t reserved[3];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
 int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);

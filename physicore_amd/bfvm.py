@@ -42,8 +42,8 @@ class DomainDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("force_lds_sweeps", C.c_int32),
-                ("x_sweep", C.c_int32), ("no_pipeline", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("x_sweep", C.c_int32), ("pipeline", C.c_int32), ("round_nearest", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 _LIB = None
@@ -274,20 +274,22 @@ class Domain:
         self._check(lib().bfvm_get_tuning(self._h, C.byref(t)))
         return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu,
                 "force_lds_sweeps": t.force_lds_sweeps, "x_sweep": t.x_sweep,
-                "no_pipeline": t.no_pipeline}
+                "pipeline": t.pipeline, "round_nearest": t.round_nearest}
 
-    def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0, x_sweep=0, no_pipeline=0):
+    def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0, x_sweep=0, pipeline=0,
+                   round_nearest=0):
         t = Tuning()
         t.persistent_wgs_per_cu = int(persistent_wgs_per_cu)
         t.force_lds_sweeps = int(force_lds_sweeps)
         t.x_sweep = int(x_sweep)
-        t.no_pipeline = int(no_pipeline)
+        t.pipeline = int(pipeline)
+        t.round_nearest = int(round_nearest)
         self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))
 
 
 def _tuning_dict(t):
     return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu, "force_lds_sweeps": t.force_lds_sweeps,
-            "x_sweep": t.x_sweep, "no_pipeline": t.no_pipeline}
+            "x_sweep": t.x_sweep, "pipeline": t.pipeline, "round_nearest": t.round_nearest}
 
 
 class Batch:

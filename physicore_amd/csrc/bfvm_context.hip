@@ -16,24 +16,21 @@
 #include "bfvm_internal.h"
 
 namespace {
-// x kernel of BFVM_X_AUTO, from x-sweep timings on MI355X (FP64 recurrences,
-// nx x 128 x 128 domains, nx 32..200): with few substrates a lane's strided
-// line touches a new cache line at every step, so the row-parallel scan wins
-// (S = 1: 20 us against 63 us strided at nx = 200); the LDS transpose wins
-// for short rows of 2 substrates; from S = 4 the strided lines win
-// (200^3 x 4: 77 us, against 81 us for the scan and 99 us for the transpose).
 // shortest y/z line the pipelined kernel is used for by default
 constexpr int kPipeMinLine = 192;
 
+// x kernel of BFVM_X_AUTO.  The row-parallel scan keeps a line's forward
+// values in FP64 registers (a lane holds at most 32 of them); the strided
+// register lines store theirs in FP32, stochastically rounded.  Timings on
+// MI355X (nx x 128 x 128, FP64 recurrences): S = 1 scan 20 us vs strided 63 us
+// at nx = 200; S = 4 scan 42 us vs strided 38 us; S = 8 scan 100 us vs
+// strided 63 us.  Short rows of 2 substrates are fastest through the LDS
+// transpose.
 int default_x_sweep(int nx, int S)
 {
-	if (S == 1)
-		return BFVM_X_SCAN;
-	if (S == 2)
-		return nx >= 96 ? BFVM_X_SCAN : BFVM_X_TRANSPOSE;
-	if (S == 3)
-		return nx >= 128 ? BFVM_X_SCAN : BFVM_X_STRIDED;
-	return BFVM_X_STRIDED;
+	if (S == 2 && nx < 96)
+		return BFVM_X_TRANSPOSE;
+	return S <= 4 ? BFVM_X_SCAN : BFVM_X_STRIDED;
 }
 } // namespace
 
@@ -158,6 +155,7 @@ struct bfvm_ctx
 	int num_cu = 256;
 
 	dvec<float> field;
+	dvec<uint32_t> step_count; // [1] steps taken (keys of the stochastic rounding)
 	dvec<double> fb[3], fe[3], fc[3];
 	dvec<int32_t> coef_map;
 	dvec<float> face_val;
@@ -267,6 +265,8 @@ int sync_domain(bfvm_ctx* c)
 {
 	DomainDev& d = c->hdom;
 	d.field = c->field.p;
+	d.step = c->step_count.p;
+	d.round_nearest = c->tuning.round_nearest;
 	d.nx = (int32_t)c->n[0];
 	d.ny = (int32_t)c->n[1];
 	d.nz = (int32_t)c->n[2];
@@ -401,12 +401,14 @@ int build_tiles(bfvm_ctx* c)
 			c->pipe[axis] = PipeGeometry {};
 			if (c->tuning.force_lds_sweeps)
 				;
-			// y/z: pipelined register lines (FP64) for long lines (shorter ones
-			// gain nothing from it on MI355X: their tiles are too short to
-			// hide the next tile's load); x only on request (its strided
-			// stores make it slower than the strided register kernel)
-			else if (c->fp64 && !c->tuning.no_pipeline && pbucket && pg.waves > 0 &&
-					 (axis == 0 ? c->tuning.x_sweep == BFVM_X_PIPELINED : n >= kPipeMinLine))
+			// pipelined register lines (FP64) on request: y/z lines of at least
+			// kPipeMinLine voxels (shorter tiles are too short to hide the next
+			// tile's load), x by its own x_sweep value.  At 200^3 x 4 on MI355X
+			// the plain register lines are faster (y 58 us, z 57 us against 61
+			// and 66 us pipelined), so neither is the default.
+			else if (c->fp64 && pbucket && pg.waves > 0 &&
+					 (axis == 0 ? c->tuning.x_sweep == BFVM_X_PIPELINED
+								: c->tuning.pipeline && n >= kPipeMinLine))
 			{
 				c->variant[axis] = kSweepPipe;
 				c->nmax[axis] = pbucket;
@@ -505,6 +507,7 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 		if (c->internalized && c->conflict)
 			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->int_tiles.p, c->n_int_tiles, c->stream));
 	}
+	TRY_HIP(c, launch_step_tick(c->ddom.p, 1, c->stream));
 	TRY_HIP(c, mark(6));
 	return BFVM_OK;
 }
@@ -1015,6 +1018,8 @@ int batch_enqueue_step(bfvm_batch* b)
 													 b->stream);
 			e != hipSuccess)
 			return b->hip_fail(e, "internalized launch");
+	if (hipError_t e = launch_step_tick(b->ddoms.p, (int)b->ctx.size(), b->stream); e != hipSuccess)
+		return b->hip_fail(e, "step tick launch");
 	return BFVM_OK;
 }
 
@@ -1130,7 +1135,8 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	c->tuning.persistent_wgs_per_cu = 0;
 	c->tuning.force_lds_sweeps = 0;
 	c->tuning.x_sweep = BFVM_X_AUTO;
-	c->tuning.no_pipeline = 0;
+	c->tuning.pipeline = 0;
+	c->tuning.round_nearest = 0;
 
 	auto bail = [&](int rc) {
 		g_error = c->err;
@@ -1151,6 +1157,11 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 
 	if ((e = c->field.alloc((size_t)c->V * c->S)) != hipSuccess)
 		return bail(c->hip_fail(e, "field allocation"));
+	{
+		const uint32_t zero = 0;
+		if ((e = c->step_count.upload(&zero, 1, c->stream)) != hipSuccess)
+			return bail(c->hip_fail(e, "step counter"));
+	}
 	{
 		std::vector<float> init(c->S);
 		for (int s = 0; s < c->S; s++)
@@ -1671,11 +1682,16 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
 	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_PIPELINED)
 		return c->fail(BFVM_ERR_INVALID, "x_sweep must be one of the bfvm_x_sweep values");
-	if (t->no_pipeline != 0 && t->no_pipeline != 1)
-		return c->fail(BFVM_ERR_INVALID, "no_pipeline must be 0 or 1");
+	if (t->pipeline != 0 && t->pipeline != 1)
+		return c->fail(BFVM_ERR_INVALID, "pipeline must be 0 or 1");
+	if (t->round_nearest != 0 && t->round_nearest != 1)
+		return c->fail(BFVM_ERR_INVALID, "round_nearest must be 0 or 1");
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	const bool resync = t->round_nearest != c->tuning.round_nearest;
 	c->tuning = *t;
-	int rc = build_tiles(c);
+	int rc = resync ? sync_domain(c) : BFVM_OK;
+	if (!rc)
+		rc = build_tiles(c);
 	if (rc)
 		return rc;
 	c->version++;
@@ -1779,12 +1795,13 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	if (!b || steps < 1)
 		return BFVM_ERR_INVALID;
 	// candidate tunings, applied to every context of the batch: the x kernel,
-	// pipelining of y/z, persistence of the register kernels
+	// pipelining of y/z, persistence of the register kernels (each context
+	// keeps its own rounding mode)
 	std::vector<bfvm_tuning> cand;
-	auto T = [](int x, int nopipe, int per_cu) {
+	auto T = [](int x, int pipeline, int per_cu) {
 		bfvm_tuning t {};
 		t.x_sweep = x;
-		t.no_pipeline = nopipe;
+		t.pipeline = pipeline;
 		t.persistent_wgs_per_cu = per_cu;
 		return t;
 	};
@@ -1793,7 +1810,12 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	cand.push_back(T(BFVM_X_TRANSPOSE, 0, 0));
 	cand.push_back(T(BFVM_X_PIPELINED, 0, 0));
 	cand.push_back(T(BFVM_X_AUTO, 1, 0));
-	cand.push_back(T(BFVM_X_AUTO, 1, 2));
+	cand.push_back(T(BFVM_X_AUTO, 0, 2));
+	auto apply = [&](bfvm_ctx* c, const bfvm_tuning& t0) {
+		bfvm_tuning t = t0;
+		t.round_nearest = c->tuning.round_nearest;
+		return bfvm_set_tuning(c, &t);
+	};
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (hipError_t e = hipEventCreate(&e0); e != hipSuccess)
 		return b->hip_fail(e, "hipEventCreate");
@@ -1807,7 +1829,7 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	for (size_t k = 0; k < cand.size() && rc == BFVM_OK; k++)
 	{
 		for (bfvm_ctx* c : b->ctx)
-			if ((rc = bfvm_set_tuning(c, &cand[k])))
+			if ((rc = apply(c, cand[k])))
 			{
 				b->err = c->err;
 				break;
@@ -1838,7 +1860,7 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	if (rc)
 		return rc;
 	for (bfvm_ctx* c : b->ctx)
-		if ((rc = bfvm_set_tuning(c, &cand[best])))
+		if ((rc = apply(c, cand[best])))
 			return b->fail(rc, c->err);
 	if (chosen)
 		*chosen = cand[best];

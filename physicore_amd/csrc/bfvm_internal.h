@@ -17,9 +17,10 @@ constexpr int kFaces = 6;		   // x-min, x-max, y-min, y-max, z-min, z-max
 struct DomainDev
 {
 	float* field; // ((z*ny + y)*nx + x)*S + s
+	uint32_t* step; // [1] steps taken: seeds the rounding keys (field_round)
 	int32_t nx, ny, nz, S, dims, L; // L = nx*S, one row
 	int32_t face_mask;				// bit f: face f has a condition for some substrate
-	int32_t pad0;
+	int32_t round_nearest;			// 1: FP64 results rounded to nearest, not stochastically
 	double dt, voxel_volume;
 	int64_t bbox_min[3];
 	int64_t voxel[3];
@@ -76,6 +77,108 @@ struct DomainDev
 	int32_t compute_internalized;
 	int32_t pad1;
 };
+
+// ---------------------------------------------------------------- rounding
+// Every FP64 result written to the FP32 field (sweep outputs, the FP32-stored
+// forward values of the register and LDS sweeps, bulk and cell updates) is
+// rounded stochastically: up with probability equal to the discarded fraction
+// of an FP32 ulp, so the rounding error has mean zero.  Round-to-nearest makes
+// the same error at a voxel every step once the field sits near a fixed point
+// of the step, and that bias accumulates as error / (per-step relaxation
+// rate): with decay 0.23 and bulk supply 0.5 (relaxation 0.0073 per step at
+// dt = 0.01) it reaches 1e-5 of the field after 1000 steps, against 2.5e-6 with
+// stochastic rounding (C2-like 64^3 emulation, substrate of D = 1.66).  This is
+// the stagnation of low-precision time stepping analysed by Croci and Giles,
+// "Effects of round-to-nearest and stochastic rounding in the numerical solution
+// of the heat equation in low precision" (IMA J. Numer. Anal., 2022).
+//
+// The random bits are a pure function of (domain step count, stage, line,
+// position), so results are reproducible run to run and identical whether a
+// domain is stepped alone or in a batch.
+constexpr uint32_t kWeyl = 0x9e3779b9u;
+
+// Rounding stages (distinct key streams within a step).
+enum RoundStage : uint32_t
+{
+	kRoundFwdX = 0,
+	kRoundOutX = 1,
+	kRoundFwdY = 2,
+	kRoundOutY = 3,
+	kRoundFwdZ = 4,
+	kRoundOutZ = 5,
+	kRoundBulk = 6,
+	kRoundCells = 7
+};
+
+// 32-bit integer finaliser ("lowbias32", C. Wellons' hash-prospector).
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+	x ^= x >> 16;
+	x *= 0x7feb352du;
+	x ^= x >> 15;
+	x *= 0x846ca68bu;
+	x ^= x >> 16;
+	return x;
+}
+
+__device__ __forceinline__ uint32_t stage_seed(const DomainDev& d, uint32_t stage)
+{
+	return mix32(*d.step * 8u + stage);
+}
+
+// FP64 -> FP32 with stochastic rounding, exact in any FP32 rounding mode: the
+// top 29 bits of `key` are added to the 29 mantissa bits that FP32 drops and
+// the value is truncated to FP32 precision (the conversion is then exact;
+// FP32 subnormals round to nearest).  Infinities and NaNs pass unchanged;
+// `nearest` selects round-to-nearest.  For the sparse stages.
+__device__ __forceinline__ float sr_round(double v, uint32_t key, bool nearest)
+{
+	const uint64_t b = __builtin_bit_cast(uint64_t, v);
+	const uint64_t up = (b + (uint64_t)(key >> 3)) & ~(uint64_t)0x1fffffff;
+	const bool special = ((b >> 52) & 0x7ff) == 0x7ff;
+	return (float)__builtin_bit_cast(double, (nearest || special) ? b : up);
+}
+
+// The sweeps' form, three VALU instructions per value: the kernel sets the
+// wave's FP32 rounding mode to round-toward-zero once (fp32_toward_zero), so
+// the conversion itself truncates after the 29 random bits r are added.  The
+// bits of element i of a line are k ^ (weyl29(i) & m): k is the line's hashed
+// key (29 bits), weyl29(i) a compile-time Weyl sequence, m a wave-uniform
+// mask, so the AND runs on the scalar unit.  Round-to-nearest (ties away from
+// zero) is k = 2^28, m = 0.  An infinite result becomes a NaN.
+__device__ __forceinline__ void fp32_toward_zero()
+{
+	__builtin_amdgcn_s_setreg(0x801, 3); // MODE[1:0] (FP32 round mode) = 3, toward zero
+}
+
+struct LineKey
+{
+	uint32_t k, m;
+};
+
+__device__ __forceinline__ LineKey line_key(const DomainDev& d, uint32_t stage, uint32_t line)
+{
+	if (d.round_nearest)
+		return { 1u << 28, 0u };
+	return { mix32(line ^ stage_seed(d, stage)) >> 3, 0x1fffffffu };
+}
+
+__host__ __device__ constexpr uint32_t weyl29(uint32_t i) { return (i * kWeyl) >> 3; }
+
+template <typename ACC>
+__device__ __forceinline__ float field_round(ACC v, LineKey key, uint32_t w)
+{
+	if constexpr (sizeof(ACC) == 8)
+	{
+		// w & m on the scalar unit: the VALU then needs one VOP2 XOR with an
+		// SGPR operand (no VGPR for the constant)
+		uint32_t wm;
+		asm("s_and_b32 %0, %1, %2" : "=s"(wm) : "s"(w), "s"(key.m) : "scc");
+		return (float)__builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) + (uint64_t)(key.k ^ wm));
+	}
+	else
+		return (float)v; // an FP32 recurrence has rounded already
+}
 
 // One tile of a sweep.
 //   x (both variants): a = first row (row = y + ny*z), b = rows in the tile (64/S)
@@ -177,6 +280,8 @@ hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* ref
 // Sparse stages over tiles (dom, first entry, count <= kSparseTile).
 constexpr int kSparseTile = 256;
 hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
+// Ends a step of `ndoms` domains: advances each one's step count.
+hipError_t launch_step_tick(const DomainDev* doms, int ndoms, hipStream_t stream);
 hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);

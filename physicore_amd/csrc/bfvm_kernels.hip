@@ -43,13 +43,24 @@ __global__ void bulk_kernel(const DomainDev* __restrict__ doms, const TileRef* _
 	const TileRef tr = tiles[blockIdx.x];
 	const DomainDev& d = doms[tr.dom];
 	const int S = d.S;
+	const uint32_t seed = stage_seed(d, kRoundBulk);
+	const bool nearest = d.round_nearest != 0;
 	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 	{
 		const int k = tr.a + t / S, s = t % S;
 		const int64_t q = (int64_t)k * S + s;
-		float* D = d.field + d.bulk_vox[k] * S + s;
-		*D = (float)(((double)*D + d.bulk_a[q]) / d.bulk_b[q]);
+		const int64_t e = d.bulk_vox[k] * S + s;
+		float* D = d.field + e;
+		*D = sr_round(((double)*D + d.bulk_a[q]) / d.bulk_b[q], mix32((uint32_t)e ^ seed), nearest);
 	}
+}
+
+// Advances the step count of every domain of a launch (the last kernel of a step).
+__global__ void step_tick_kernel(const DomainDev* __restrict__ doms, int ndoms)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ndoms)
+		doms[i].step[0] += 1u;
 }
 
 // ---------------------------------------------------------------- cells
@@ -291,14 +302,17 @@ __global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const Til
 	const DomainDev& d = doms[tr.dom];
 	const int S = d.S;
 	const bool fused = d.compute_internalized && !d.flags[0];
+	const uint32_t seed = stage_seed(d, kRoundCells);
+	const bool nearest = d.round_nearest != 0;
 	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 	{
 		const int k = tr.a + t / S, s = t % S;
 		const int64_t q = (int64_t)k * S + s;
-		float* D = d.field + (int64_t)d.pvox[k] * S + s;
+		const int64_t e = (int64_t)d.pvox[k] * S + s;
+		float* D = d.field + e;
 		const double d0 = (double)*D;
 		const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
-		*D = (float)d1;
+		*D = sr_round(d1, mix32((uint32_t)e ^ seed), nearest);
 		if (fused)
 			d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
 		else
@@ -592,6 +606,14 @@ hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, 
 	if (ntiles <= 0)
 		return hipSuccess;
 	hipLaunchKernelGGL(bulk_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
+	return hipGetLastError();
+}
+
+hipError_t launch_step_tick(const DomainDev* doms, int ndoms, hipStream_t stream)
+{
+	if (ndoms <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(step_tick_kernel, dim3((ndoms + 63) / 64), dim3(64), 0, stream, doms, ndoms);
 	return hipGetLastError();
 }
 

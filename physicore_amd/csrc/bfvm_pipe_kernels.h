@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 	int first = blockIdx.x * W;
 	if (first >= ntiles)
 		return;
+	fp32_toward_zero();
 
 	// stale buffer contents only ever feed padding steps (b' = 0 there), so
 	// they must be finite: clear once
@@ -166,8 +167,10 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 		const __amdgpu_buffer_rsrc_t rsrc = field_rsrc(d);
 		const int S = AXIS == 0 ? SX : d.S, L = d.L;
 
-		// this lane's line in the tile
+		// this lane's line in the tile (line: its rounding-key index, as in the
+		// register kernels)
 		int slot, cs, voff, stride4;
+		uint32_t line;
 		line_faces lf {};
 		if constexpr (AXIS == 0)
 		{
@@ -176,6 +179,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 			const int s = lane / S < rows ? lane - (lane / S) * S : S - 1;
 			slot = r * pipe_x_pitch(L, S) + s;
 			cs = s;
+			line = (uint32_t)((tr.a + r) * S + s);
 			voff = ((tr.a + r) * L + s) * (int)sizeof(float);
 			stride4 = S * (int)sizeof(float);
 			if constexpr (FACES)
@@ -187,6 +191,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 			const int total = pipe_total<AXIS>(d);
 			const int f = min(tr.a + lane, total - 1);
 			slot = f - tr.a;
+			line = (uint32_t)f;
 			const int outer = f / L, c = f - outer * L;
 			cs = d.coef_map ? d.coef_map[c] : c % S;
 			voff = pipe_col_off<AXIS>(d, f, total);
@@ -226,6 +231,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 		}
 		if (tr.a < 0)
 			continue;
+		const LineKey kf = line_key(d, 2 * AXIS, line);
+		const LineKey ko = line_key(d, 2 * AXIS + 1, line);
 
 		const double* lb = fb + cs * NMAX;
 		const double cc = fc[cs];
@@ -256,7 +263,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 				if constexpr (i < NMAX)
 				{
 					prev = (double)v[i] + (cc * fa[C & 1][K]) * prev;
-					v[i] = (float)prev;
+					v[i] = field_round<double>(prev, kf, weyl29(i));
 				}
 			});
 			__builtin_amdgcn_sched_barrier(0);
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 		const double btop = lb[NMAX - 1];
 		__builtin_amdgcn_sched_barrier(0);
 		double nxt = (double)v[NMAX - 1] * btop;
-		out(std::integral_constant<int, NMAX - 1> {}, (float)nxt);
+		out(std::integral_constant<int, NMAX - 1> {}, field_round<double>(nxt, ko, weyl29(NMAX - 1)));
 		static_for<nchunks>([&](auto C) {
 			if constexpr (C + 1 < nchunks)
 				load_bwd(std::integral_constant<int, C + 1> {}, fa[(C + 1) & 1]);
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 				if constexpr (i >= 0)
 				{
 					nxt = ((double)v[i] + cc * nxt) * fa[C & 1][K];
-					out(std::integral_constant<int, i> {}, (float)nxt);
+					out(std::integral_constant<int, i> {}, field_round<double>(nxt, ko, weyl29(i)));
 				}
 			});
 			__builtin_amdgcn_sched_barrier(0);

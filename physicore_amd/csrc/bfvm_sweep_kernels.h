@@ -115,7 +115,9 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int voff, in
 // length are padding, decoupled by the zero factors).  No branches: the whole
 // solve is one basic block, so the loads feeding it can all be in flight.
 // `out(i, value)` receives the results from i = NMAX-1 down to 0; fb/fe point
-// at this line's substrate row.
+// at this line's substrate row.  kf / ko: the line's rounding keys for the
+// forward values kept in v and for the results (FP64: the caller has set the
+// FP32 rounding mode to toward-zero).
 // The recurrences run in chunks of kChunk steps separated by scheduling
 // barriers, so the compiler neither hoists every factor read nor every store
 // offset to the top of the (fully unrolled) solve, which would exhaust VGPRs
@@ -124,7 +126,7 @@ constexpr int kChunk = 16;
 
 template <int NMAX, typename ACC, typename Out>
 __device__ __forceinline__ void thomas_regs(float (&v)[NMAX], const double* __restrict__ fb,
-											const double* __restrict__ fe, double c, Out out)
+											const double* __restrict__ fe, double c, LineKey kf, LineKey ko, Out out)
 {
 	constexpr int nchunks = (NMAX - 1 + kChunk - 1) / kChunk;
 	ACC prev = (ACC)v[0];
@@ -134,21 +136,21 @@ __device__ __forceinline__ void thomas_regs(float (&v)[NMAX], const double* __re
 			if constexpr (i < NMAX)
 			{
 				prev = (ACC)v[i] + (ACC)fe[i - 1] * prev;
-				v[i] = (float)prev;
+				v[i] = field_round<ACC>(prev, kf, weyl29(i));
 			}
 		});
 		__builtin_amdgcn_sched_barrier(0);
 	});
 	const ACC cc = (ACC)c;
 	ACC nxt = (ACC)v[NMAX - 1] * (ACC)fb[NMAX - 1];
-	out(NMAX - 1, (float)nxt);
+	out(NMAX - 1, field_round<ACC>(nxt, ko, weyl29(NMAX - 1)));
 	static_for<nchunks>([&](auto C) {
 		static_for<kChunk>([&](auto K) {
 			constexpr int i = NMAX - 2 - (C * kChunk + K);
 			if constexpr (i >= 0)
 			{
 				nxt = ((ACC)v[i] + cc * nxt) * (ACC)fb[i];
-				out(i, (float)nxt);
+				out(i, field_round<ACC>(nxt, ko, weyl29(i)));
 			}
 		});
 		__builtin_amdgcn_sched_barrier(0);
@@ -237,6 +239,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 	double* fe = lds + fac_cap;
 	double* fc = lds + 2 * fac_cap;
 	int cur = -1;
+	if constexpr (sizeof(ACC) == 8)
+		fp32_toward_zero();
 
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
 	{
@@ -264,6 +268,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 			(AXIS == 0 ? outer * L + c : (AXIS == 1 ? outer * d.ny * L + c : f)) * (int)sizeof(float);
 		const int cs = AXIS == 0 ? c : (d.coef_map ? d.coef_map[c] : c % S);
 		const __amdgpu_buffer_rsrc_t rsrc = field_rsrc(d);
+		// line f of this axis: (row, s) for x, (outer, x*S + s) for y and z
+		const LineKey kf = line_key(d, 2 * AXIS, (uint32_t)f);
+		const LineKey ko = line_key(d, 2 * AXIS + 1, (uint32_t)f);
 
 		float v[NMAX];
 		static_for<NMAX>([&](auto I) { v[I] = buf_load(rsrc, voff, min((int)I, n - 1) * stride4); });
@@ -287,14 +294,14 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 					lf = AXIS == 1 ? faces_of_line(d, 1, x, 0, outer, s) : faces_of_line(d, 2, x, outer, 0, s);
 				}
 			}
-			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], [&](int i, float val) {
+			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko, [&](int i, float val) {
 				const int k = min(i, sn);
 				buf_store(rsrc, voff, k * sstride4, face_value(lf, k, sn + 1, val));
 			});
 		}
 		else
 		{
-			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs],
+			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
 								   [&](int i, float val) { buf_store(rsrc, voff, min(i, sn) * sstride4, val); });
 		}
 	}
@@ -316,6 +323,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_rows_kernel(const DomainD
 	float* tile = reinterpret_cast<float*>(lds + 2 * fac_cap + 64);
 	const int tid = threadIdx.x;
 	int cur = -1;
+	if constexpr (sizeof(ACC) == 8)
+		fp32_toward_zero();
 
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
 	{
@@ -360,8 +369,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_rows_kernel(const DomainD
 			__builtin_amdgcn_sched_barrier(0);
 			int sn = n - 1, sS = S;
 			asm volatile("" : "+s"(sn), "+s"(sS));
-			thomas_regs<NMAX, ACC>(v, fb + s * NMAX, fe + s * NMAX, fc[s],
-								   [&](int i, float val) { line[min(i, sn) * sS] = val; });
+			const uint32_t f = (uint32_t)((tr.a + r) * S + s); // line (row, s)
+			thomas_regs<NMAX, ACC>(v, fb + s * NMAX, fe + s * NMAX, fc[s], line_key(d, kRoundFwdX, f),
+								   line_key(d, kRoundOutX, f), [&](int i, float val) { line[min(i, sn) * sS] = val; });
 		}
 		__syncthreads();
 		const bool faces = finish && d.face_mask && d.dims == 1;
@@ -399,24 +409,26 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_rows_kernel(const DomainD
 // columns of one plane: tr.a = outer, tr.b = first column).
 template <int AXIS, typename ACC>
 __device__ __forceinline__ void thomas_lds(float* line, int stride, int n, const double* fb, const double* fe,
-										   double c)
+										   double c, const DomainDev& d, uint32_t f)
 {
+	const LineKey kf = line_key(d, 2 * AXIS, f);
+	const LineKey ko = line_key(d, 2 * AXIS + 1, f);
 	ACC prev = (ACC)line[0];
 	for (int i = 1; i < n; i++)
 	{
 		float* p = line + i * stride;
 		const ACC v = (ACC)(*p) + (ACC)fe[i - 1] * prev;
-		*p = (float)v;
+		*p = field_round<ACC>(v, kf, weyl29(i));
 		prev = v;
 	}
 	ACC nxt = prev * (ACC)fb[n - 1];
-	line[(n - 1) * stride] = (float)nxt;
+	line[(n - 1) * stride] = field_round<ACC>(nxt, ko, weyl29(n - 1));
 	const ACC cc = (ACC)c;
 	for (int i = n - 2; i >= 0; i--)
 	{
 		float* p = line + i * stride;
 		nxt = ((ACC)(*p) + cc * nxt) * (ACC)fb[i];
-		*p = (float)nxt;
+		*p = field_round<ACC>(nxt, ko, weyl29(i));
 	}
 }
 
@@ -432,6 +444,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_lds_kernel(const DomainDe
 	float* tile = reinterpret_cast<float*>(lds + 2 * fac_cap + 64);
 	const int tid = threadIdx.x;
 	int cur = -1;
+	if constexpr (sizeof(ACC) == 8)
+		fp32_toward_zero();
 
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
 	{
@@ -460,7 +474,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_lds_kernel(const DomainDe
 			if (tid < rows * S)
 			{
 				const int r = tid / S, s = tid - (tid / S) * S;
-				thomas_lds<AXIS, ACC>(tile + r * Lp + s, S, n, fb + s * n, fe + s * n, fc[s]);
+				thomas_lds<AXIS, ACC>(tile + r * Lp + s, S, n, fb + s * n, fe + s * n, fc[s], d,
+									  (uint32_t)((tr.a + r) * S + s));
 			}
 			__syncthreads();
 			for (int e = tid; e < total; e += kSweepThreads)
@@ -492,7 +507,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_lds_kernel(const DomainDe
 			{
 				const int c = c0 + tid;
 				const int cs = d.coef_map ? d.coef_map[c] : c % S;
-				thomas_lds<AXIS, ACC>(tile + tid, kLinesPerTile, n, fb + cs * n, fe + cs * n, fc[cs]);
+				thomas_lds<AXIS, ACC>(tile + tid, kLinesPerTile, n, fb + cs * n, fe + cs * n, fc[cs], d,
+									  (uint32_t)(tr.a * L + c));
 			}
 			__syncthreads();
 			if (j < w)
@@ -638,6 +654,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const Domain
 	int cur = -1;
 	int nx = 0;
 	ACC fa[C], fbv[C], cs[S];
+	if constexpr (sizeof(ACC) == 8)
+		fp32_toward_zero();
 
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
 	{
@@ -704,12 +722,13 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const Domain
 			// carry into this lane = inclusive result of the lane before (0 for lane 0)
 			B[s] = dpp_mov<0x138, 0xf>((ACC)0, B[s]);
 		});
-		// forward results overwrite the loaded values (FP32, as the register
-		// kernels keep theirs)
+		// forward results kept in ACC (a lane holds only C of them), so only
+		// the results are rounded to FP32
+		ACC fw[C];
 		static_for<C>([&](auto K) {
 			constexpr int s = K % S;
 			B[s] = fa[K] * B[s] + (ACC)v[K];
-			v[K] = (float)B[s];
+			fw[K] = B[s];
 		});
 
 		// backward
@@ -721,7 +740,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const Domain
 			constexpr int j = VPL - 1 - K / S, s = K % S;
 			constexpr int e = j * S + s;
 			const ACC cb = cs[s] * fbv[e];
-			B[s] = cb * B[s] + fbv[e] * (ACC)v[e];
+			B[s] = cb * B[s] + fbv[e] * fw[e];
 			A[s] = cb * A[s];
 		});
 		static_for<S>([&](auto s) {
@@ -729,11 +748,13 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xscan_kernel(const Domain
 			// carry from the right = inclusive result of the next lane (0 past lane 63)
 			B[s] = dpp_mov<0x130, 0xf>((ACC)0, B[s]);
 		});
+		// one key per (row, lane); element e of the lane adds weyl29(e)
+		const LineKey ko = line_key(d, kRoundOutX, (uint32_t)(row * 64 + lane));
 		static_for<C>([&](auto K) {
 			constexpr int j = VPL - 1 - K / S, s = K % S;
 			constexpr int e = j * S + s;
-			B[s] = cs[s] * fbv[e] * B[s] + fbv[e] * (ACC)v[e];
-			v[e] = (float)B[s];
+			B[s] = cs[s] * fbv[e] * B[s] + fbv[e] * fw[e];
+			v[e] = field_round<ACC>(B[s], ko, weyl29(e));
 		});
 
 		if (vec && full)

@@ -21,7 +21,7 @@ import scenarios as sc  # noqa: E402
 from gpu_helpers import gpu_domain, rel_linf  # noqa: E402
 
 
-def run_case(name, scn, steps, internalized=False, reduces=("atomic", "sorted")):
+def run_case(name, scn, steps, internalized=False, reduces=("atomic", "sorted"), tunings=(None,)):
     p, field, dirs, bulk, ag = scn[:5]
     f0 = field.astype(np.float32).astype(np.float64)
     t0 = time.perf_counter()
@@ -32,23 +32,39 @@ def run_case(name, scn, steps, internalized=False, reduces=("atomic", "sorted"))
     t_cpu = time.perf_counter() - t0
     out = {"case": name, "steps": steps, "grid": [p.nx, p.ny, p.nz], "S": p.S,
            "cells": 0 if ag is None else int(ag.n), "oracle_s": round(t_cpu, 2)}
-    for prec in ("fp64", "fp32"):
-        for red in reduces:
-            d = gpu_domain(p, field=field, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized,
-                           precision=prec, reduce=red)
-            d.step_graph(steps, min(10, steps))
-            got = d.field()
-            out[f"{prec}_{red}_rel_linf"] = [float(x) for x in rel_linf(got, want)]
-            if prec == "fp64" and red == reduces[0]:
-                V = float(np.prod(p.voxel))
-                out["mass_gpu"] = [float(x) for x in d.total_mass()]
-                out["mass_oracle"] = [float(x) for x in (want.reshape(-1, p.S).sum(axis=0) * V)]
+    V = float(np.prod(p.voxel))
+    for k, tuning in enumerate(tunings):
+        tag = "" if tuning is None else f"_t{k}"
+        if tuning is not None:
+            out[f"tuning{tag}"] = tuning
+        for prec in ("fp64", "fp32"):
+            for red in reduces:
+                d = gpu_domain(p, field=field, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized,
+                               precision=prec, reduce=red)
+                if tuning:
+                    d.set_tuning(**tuning)
+                d.step_graph(steps, min(10, steps))
+                got = d.field()
+                out[f"{prec}_{red}_rel_linf{tag}"] = [float(x) for x in rel_linf(got, want)]
+                if prec == "fp64" and red == reduces[0]:
+                    # where the largest error sits: (z, y, x) and the FP64 value there
+                    err = np.abs(got.reshape(-1, p.S) - want.reshape(-1, p.S))
+                    at = err.argmax(axis=0)
+                    out[f"worst_voxel_zyx{tag}"] = [[int(i // (p.nx * p.ny)), int(i // p.nx % p.ny),
+                                                     int(i % p.nx)] for i in at]
+                    out[f"worst_value{tag}"] = [float(want.reshape(-1, p.S)[i, s]) for s, i in enumerate(at)]
+                    out[f"mass_gpu{tag}"] = [float(x) for x in d.total_mass()]
+    out["mass_oracle"] = [float(x) for x in (want.reshape(-1, p.S).sum(axis=0) * V)]
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true", help="fewer steps on the large cases")
+    ap.add_argument("--only", default="", help="run only the cases whose name contains this text")
+    ap.add_argument("--tuning", action="append", default=[],
+                    help='kernel selection for the GPU runs, as JSON (keyword arguments of '
+                    'Domain.set_tuning), e.g. \'{"round_nearest": 1}\'; may be repeated, "null" = the default')
     a = ap.parse_args()
     q = a.quick
     cases = []
@@ -62,7 +78,9 @@ def main():
                       20 if q else 500))
     cases.append(("config4 400x400x100x2, 2e6 cells", sc.config4(), 3 if q else 200))
     for name, scn, steps in cases:
-        print(json.dumps(run_case(name, scn, steps)), flush=True)
+        if a.only in name:
+            tunings = [json.loads(t) for t in a.tuning] or [None]
+            print(json.dumps(run_case(name, scn, steps, tunings=tunings)), flush=True)
 
 
 if __name__ == "__main__":

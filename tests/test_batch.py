@@ -131,7 +131,7 @@ def test_batch_autotune_keeps_results():
     ref = [_gpu(s) for s in scns]
     b = bfvm.Batch(doms)
     t = b.autotune(steps=2)
-    assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "no_pipeline"}
+    assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "pipeline", "round_nearest"}
     # the autotuner advanced the domains by 6 candidates x (2 + 2) steps
     for d in ref:
         d.step(24)

@@ -62,7 +62,7 @@ SHAPES = [
 # allows), each explicit x kernel (strided register lines, LDS transpose,
 # affine scan), no pipelining, and every line solved inside an LDS tile
 VARIANTS = {"default": {}, "xstrided": {"x_sweep": 1}, "xrows": {"x_sweep": 2}, "xscan": {"x_sweep": 3},
-            "xpipe": {"x_sweep": 4}, "nopipe": {"no_pipeline": 1}, "lds": {"force_lds_sweeps": 1}}
+            "xpipe": {"x_sweep": 4}, "pipe": {"pipeline": 1}, "nearest": {"round_nearest": 1}, "lds": {"force_lds_sweeps": 1}}
 
 
 @pytest.mark.parametrize("dims,grid,S", SHAPES)
@@ -91,6 +91,58 @@ def test_fp32_arithmetic_mode_runs():
     d = gpu_domain(p, field=f0, precision="fp32")
     d.step(1)
     assert rel_linf(d.field(), want).max() < 1e-5
+
+
+def _fp32_neighbours(x):
+    """The FP32 values just below and above each FP64 value (x not representable)."""
+    lo = x.astype(np.float32)
+    lo = np.where(lo.astype(np.float64) > x, np.nextafter(lo, np.float32(-np.inf)), lo)
+    return lo, np.nextafter(lo, np.float32(np.inf))
+
+
+def test_stochastic_rounding_distribution():
+    """With D = 0 a 1-D step is one FP64 product c0*b per voxel, rounded once
+    to FP32 (lines of 1000: the LDS-tile sweep).  Stochastic rounding puts
+    each value on one of its two FP32 neighbours and rounds up with
+    probability equal to the discarded fraction of the ulp; round_nearest
+    gives numpy's FP32 rounding."""
+    lam = np.array([0.7, 0.3, 0.9, 0.5])
+    p = orc.Problem(1, (1000, 1, 1), (20, 20, 20), 0.01, np.zeros(4), lam)
+    c0 = np.float32(0.3)
+    field = np.full(p.shape, float(c0))
+    exact = float(c0) / (1 + p.dt * lam)  # b of a decay-only line, dims = 1
+    lo, hi = _fp32_neighbours(exact)
+    frac = (exact - lo.astype(np.float64)) / (hi.astype(np.float64) - lo.astype(np.float64))
+    d = gpu_domain(p, field=field)
+    d.step(1)
+    f = d.field().reshape(-1, 4)
+    for s in range(4):
+        assert np.all((f[:, s] == lo[s]) | (f[:, s] == hi[s])), s
+        up = np.mean(f[:, s] == hi[s])
+        assert abs(up - frac[s]) < 5 * np.sqrt(frac[s] * (1 - frac[s]) / len(f)) + 1e-3, (s, up, frac[s])
+    d = gpu_domain(p, field=field)
+    d.set_tuning(round_nearest=1)
+    d.step(1)
+    assert np.array_equal(d.field().reshape(-1, 4), np.broadcast_to(exact.astype(np.float32), (1000, 4)))
+
+
+def test_stochastic_rounding_unbiased_3d():
+    """Three roundings per step (x scan, y and z register lines): E[field] is
+    the exact c0*b^3, so the voxel mean matches it to a few standard errors
+    (round-to-nearest is off by a fixed fraction of an ulp), and every voxel is
+    within 3 ulps of it."""
+    lam = 0.7
+    p = orc.Problem(3, (64, 64, 64), (20, 20, 20), 0.01, [0.0], [lam])
+    c0 = np.float32(0.3)
+    exact = float(c0) / (1 + p.dt * lam / 3) ** 3
+    ulp = float(np.spacing(np.float32(exact)))
+    d = gpu_domain(p, field=np.full(p.shape, float(c0)))
+    d.step(1)
+    f = d.field().astype(np.float64).ravel()
+    assert np.abs(f - exact).max() <= 3 * ulp
+    assert len(np.unique(f)) > 1
+    sigma = 0.5 * np.sqrt(3) * ulp / np.sqrt(f.size)
+    assert abs(f.mean() - exact) < 6 * sigma, (f.mean() - exact) / ulp
 
 
 def test_pure_diffusion_mass_balance():
