@@ -24,6 +24,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sweeps_200cube_s4_hbm_traffic.json")
+
+
+def measured_traffic(kernel, side, S):
+    """HBM bytes per launch of `kernel` (sweep_x/y/z) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload, each counter
+    scaled by what it reads for a copy of known size; None for other shapes."""
+    if (side, S) != (200, 4) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    prof = json.load(open(TRAFFIC_PROFILE))
+    k = prof["kernels"].get(kernel)
+    if k is None:
+        return None
+    kib = k["fetch_kib"] * prof["fetch_factor"] + k["write_kib"] * prof["write_factor"]
+    return {"bytes": kib * 1024, "kernel_symbol": k["kernel"], "source": "profiles/" + prof["source"]}
 
 
 def build_c2(side, S, n_cells, seed):
@@ -258,6 +273,7 @@ def main():
     dominant = max(("x", "y", "z"), key=lambda k: stages[k])
     achieved = sweep_bytes / (stages[dominant] * 1e-6) / 1e9
     diff_us = stages["x"] + stages["y"] + stages["z"]
+    traffic = measured_traffic(f"sweep_{dominant}", args.side, args.substrates)
     step_gbs = units * 24 / (ms_per_step * 1e-3) / 1e9
 
     out = {
@@ -279,7 +295,8 @@ def main():
                    "graph_steps": args.graph_steps, "seed": 43},
         "roofline": {"bound": "hbm", "kernel": f"sweep_{dominant}", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic,
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "avg_launch_us": stages[dominant]},
         "step_roofline": {"bytes_per_step": units * 24, "achieved": step_gbs,

@@ -152,6 +152,27 @@ int bfvm_upload_field_f64(bfvm_ctx* ctx, const double* host);
 int bfvm_download_field_f32(bfvm_ctx* ctx, float* host);
 int bfvm_upload_field_f32(bfvm_ctx* ctx, const float* host);
 
+/* Snapshots of the field for output while stepping goes on (replaces the
+ * per-voxel get_substrate_density loop of vtk_serializer::serialize,
+ * BFVM/src/vtk_serializer.cpp:54-79).  bfvm_snapshot_begin queues, after the
+ * steps queued so far, a device-to-device copy of the FP32 field into a
+ * snapshot buffer of the context (allocated on first use; the copy runs at
+ * HBM speed on the context stream), then the copy of that snapshot to `host`
+ * (nx*ny*nz*S floats, layout as above) on a separate copy stream, and returns
+ * a ticket.  Steps queued afterwards run while the host copy is in flight
+ * when `host` is pinned (bfvm_host_alloc); pageable memory is copied
+ * synchronously.  A snapshot begun while the previous one is still being
+ * copied makes the context stream wait for that copy.  bfvm_snapshot_wait
+ * blocks until snapshot `ticket` is in `host`; it may be called from another
+ * thread than the one stepping the context (its errors, and those of
+ * bfvm_host_alloc/free, are reported by bfvm_last_error(NULL)). */
+int bfvm_snapshot_begin(bfvm_ctx* ctx, float* host, int64_t* ticket);
+int bfvm_snapshot_wait(bfvm_ctx* ctx, int64_t ticket);
+
+/* Pinned host memory for bfvm_snapshot_begin. */
+int bfvm_host_alloc(size_t bytes, void** out);
+int bfvm_host_free(void* p);
+
 /* Internalized substrates of every agent, [n*S] FP64. */
 int bfvm_download_internalized(bfvm_ctx* ctx, double* host);
 

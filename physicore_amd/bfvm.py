@@ -65,6 +65,10 @@ _SIGS = {
     "bfvm_upload_field_f64": ([C.c_void_p, f64p], C.c_int),
     "bfvm_download_field_f32": ([C.c_void_p, f32p], C.c_int),
     "bfvm_upload_field_f32": ([C.c_void_p, f32p], C.c_int),
+    "bfvm_snapshot_begin": ([C.c_void_p, f32p, C.POINTER(C.c_int64)], C.c_int),
+    "bfvm_snapshot_wait": ([C.c_void_p, C.c_int64], C.c_int),
+    "bfvm_host_alloc": ([C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "bfvm_host_free": ([C.c_void_p], C.c_int),
     "bfvm_download_internalized": ([C.c_void_p, f64p], C.c_int),
     "bfvm_release_internalized": ([C.c_void_p, C.c_int64], C.c_int),
     "bfvm_total_mass": ([C.c_void_p, f64p], C.c_int),
@@ -115,6 +119,30 @@ def _p(a, t=f64p):
 def _f64(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a if shape is None else a.reshape(shape)
+
+
+class PinnedField:
+    """Pinned host memory (bfvm_host_alloc) shaped like a Domain's FP32 field,
+    the destination of Domain.snapshot_begin."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        n = int(np.prod(self.shape))
+        self._ptr = C.c_void_p()
+        rc = lib().bfvm_host_alloc(n * 4, C.byref(self._ptr))
+        if rc != OK:
+            raise BfvmError(rc, lib().bfvm_last_error(None).decode())
+        buf = (C.c_float * n).from_address(self._ptr.value)
+        self.array = np.ctypeslib.as_array(buf).reshape(self.shape)
+
+    def close(self):
+        if getattr(self, "_ptr", None) and self._ptr.value:
+            self.array = None
+            lib().bfvm_host_free(self._ptr)
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 class Domain:
@@ -223,6 +251,21 @@ class Domain:
             out = np.empty(self.shape, dtype=np.float64)
             self._check(lib().bfvm_download_field_f64(self._h, _p(out)))
         return out
+
+    def snapshot_begin(self, dst):
+        """Queues a snapshot of the field (after the steps queued so far) into
+        `dst`, a PinnedField; later steps run during its host copy.  Returns
+        the ticket for snapshot_wait."""
+        if tuple(dst.shape) != tuple(self.shape):
+            raise ValueError("snapshot buffer shape mismatch")
+        t = C.c_int64()
+        self._check(lib().bfvm_snapshot_begin(self._h, _p(dst.array, f32p), C.byref(t)))
+        return t.value
+
+    def snapshot_wait(self, ticket):
+        rc = lib().bfvm_snapshot_wait(self._h, int(ticket))
+        if rc != OK:
+            raise BfvmError(rc, lib().bfvm_last_error(None).decode())
 
     def upload_field(self, arr):
         a = np.ascontiguousarray(arr)

@@ -70,9 +70,9 @@ def build_hip(jobs=None):
 
 
 HOST_SRCS = ["mesh.cpp", "agent_data.cpp", "microenvironment.cpp", "microenvironment_builder.cpp",
-             "solver_registry.cpp", "hip_solver.cpp"]
+             "solver_registry.cpp", "hip_solver.cpp", "vtk_serializer.cpp"]
 CXX = os.environ.get("CXX", "g++")
-HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 
 def build_host():

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,17 +21,18 @@ namespace {
 constexpr int kPipeMinLine = 192;
 
 // x kernel of BFVM_X_AUTO.  The row-parallel scan keeps a line's forward
-// values in FP64 registers (a lane holds at most 32 of them); the strided
-// register lines store theirs in FP32, stochastically rounded.  Timings on
-// MI355X (nx x 128 x 128, FP64 recurrences): S = 1 scan 20 us vs strided 63 us
-// at nx = 200; S = 4 scan 42 us vs strided 38 us; S = 8 scan 100 us vs
-// strided 63 us.  Short rows of 2 substrates are fastest through the LDS
-// transpose.
+// values in FP64 registers and handles rows up to 64*32/S voxels; the strided
+// register lines (lines up to 256) store theirs in FP32, stochastically
+// rounded.  x-sweep times on MI355X (nx x 128 x 128, FP64 recurrences,
+// strided / LDS transpose / scan, us): S = 1, nx = 200: 64.6 / 38.4 / 21.1;
+// S = 2, nx = 64: 23.5 / 18.9 / 22.9; S = 2, nx = 200: 58.9 / 40.0 / 25.9;
+// S = 4, nx = 64: 16.0 / 20.3 / 33.2; S = 4, nx = 200: 38.8 / 50.5 / 41.0;
+// S = 8, nx = 200: 63.9 / 111 / 98.6.  200^3 x 4: strided 76.5, scan 80.7.
 int default_x_sweep(int nx, int S)
 {
-	if (S == 2 && nx < 96)
-		return BFVM_X_TRANSPOSE;
-	return S <= 4 ? BFVM_X_SCAN : BFVM_X_STRIDED;
+	if (S <= 2 || nx > bfvm::kRegMaxN)
+		return (S == 2 && nx < 96) ? BFVM_X_TRANSPOSE : BFVM_X_SCAN;
+	return BFVM_X_STRIDED;
 }
 } // namespace
 
@@ -206,6 +208,15 @@ struct bfvm_ctx
 
 	dvec<double> mass_partial, mass_out;
 
+	// field snapshots for output: device copy, then host copy on copy_stream;
+	// ticket t completes with snap_done[t % kSnapRing]
+	static constexpr int kSnapRing = 4;
+	dvec<float> snap;
+	hipStream_t copy_stream = nullptr;
+	hipEvent_t snap_taken = nullptr;
+	hipEvent_t snap_done[kSnapRing] = {};
+	std::atomic<int64_t> snap_next { 0 };
+
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
 	int64_t graph_steps = 0;
@@ -236,7 +247,16 @@ struct bfvm_ctx
 	{
 		if (stream)
 			(void)hipStreamSynchronize(stream);
+		if (copy_stream)
+			(void)hipStreamSynchronize(copy_stream);
 		drop_graph();
+		for (hipEvent_t e : snap_done)
+			if (e)
+				(void)hipEventDestroy(e);
+		if (snap_taken)
+			(void)hipEventDestroy(snap_taken);
+		if (copy_stream)
+			(void)hipStreamDestroy(copy_stream);
 		if (stream)
 			(void)hipStreamDestroy(stream);
 	}
@@ -1477,6 +1497,74 @@ int bfvm_download_field_f32(bfvm_ctx* c, float* host)
 	FLUSH_RELEASES(c);
 	TRY_HIP(c, hipMemcpyAsync(host, c->field.p, c->field.n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_snapshot_begin(bfvm_ctx* c, float* host, int64_t* ticket)
+{
+	if (!c || !host || !ticket)
+		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
+	if (!c->copy_stream)
+	{
+		TRY_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+		TRY_HIP(c, hipEventCreateWithFlags(&c->snap_taken, hipEventDisableTiming));
+		for (hipEvent_t& e : c->snap_done)
+			TRY_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	}
+	if (!c->snap.p)
+		TRY_HIP(c, c->snap.alloc(c->field.n));
+	const int64_t t = c->snap_next.load();
+	const size_t bytes = c->field.n * sizeof(float);
+	// the snapshot buffer is reused: the previous host copy must have read it
+	if (t > 0)
+		TRY_HIP(c, hipStreamWaitEvent(c->stream, c->snap_done[(t - 1) % bfvm_ctx::kSnapRing], 0));
+	// a kernel copy at HBM speed (a DMA-engine copy would hold the stream far longer)
+	TRY_HIP(c, launch_copy_field(c->field.p, c->snap.p, (int64_t)c->field.n, c->num_cu, c->stream));
+	TRY_HIP(c, hipEventRecord(c->snap_taken, c->stream));
+	TRY_HIP(c, hipStreamWaitEvent(c->copy_stream, c->snap_taken, 0));
+	TRY_HIP(c, hipMemcpyAsync(host, c->snap.p, bytes, hipMemcpyDeviceToHost, c->copy_stream));
+	TRY_HIP(c, hipEventRecord(c->snap_done[t % bfvm_ctx::kSnapRing], c->copy_stream));
+	c->snap_next.store(t + 1);
+	*ticket = t;
+	return BFVM_OK;
+}
+
+int bfvm_snapshot_wait(bfvm_ctx* c, int64_t ticket)
+{
+	if (!c || ticket < 0 || ticket >= c->snap_next.load())
+		return BFVM_ERR_INVALID;
+	// a later snapshot may have re-recorded the slot: waiting for it is later still
+	if (hipError_t e = hipEventSynchronize(c->snap_done[ticket % bfvm_ctx::kSnapRing]); e != hipSuccess)
+	{
+		g_error = std::string("hipEventSynchronize: ") + hipGetErrorString(e);
+		return BFVM_ERR_HIP;
+	}
+	return BFVM_OK;
+}
+
+int bfvm_host_alloc(size_t bytes, void** out)
+{
+	if (!out || bytes == 0)
+		return BFVM_ERR_INVALID;
+	*out = nullptr;
+	if (hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault); e != hipSuccess)
+	{
+		g_error = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+		return e == hipErrorOutOfMemory ? BFVM_ERR_NO_MEMORY : BFVM_ERR_HIP;
+	}
+	return BFVM_OK;
+}
+
+int bfvm_host_free(void* p)
+{
+	if (!p)
+		return BFVM_OK;
+	if (hipError_t e = hipHostFree(p); e != hipSuccess)
+	{
+		g_error = std::string("hipHostFree: ") + hipGetErrorString(e);
+		return BFVM_ERR_HIP;
+	}
 	return BFVM_OK;
 }
 

@@ -285,6 +285,7 @@ hipError_t launch_step_tick(const DomainDev* doms, int ndoms, hipStream_t stream
 hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
+hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream);
 hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);
 hipError_t launch_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream);
 hipError_t launch_release_flush(const DomainDev* dom, const int64_t* groups, int64_t n_groups, const int64_t* agents,

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace bfvm {
 
 namespace {
@@ -376,6 +378,18 @@ __global__ void fill_field_kernel(float* __restrict__ f, const float* __restrict
 		f[k] = init[k % S];
 }
 
+// Field snapshot: grid-stride copy in 16-byte pieces (both buffers come from
+// hipMalloc, so they are aligned), the last n % 4 floats by block 0.
+__global__ void copy_field_kernel(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n4,
+								  const float* __restrict__ src_tail, float* __restrict__ dst_tail, int tail)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4; k += stride)
+		dst[k] = src[k];
+	if (blockIdx.x == 0 && (int)threadIdx.x < tail)
+		dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
 __global__ void f64_to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n)
 {
 	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -636,6 +650,19 @@ hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream)
 {
 	return launch1d(cells_pack_kernel, n_owners, 256, stream, dom, n_owners);
+}
+
+hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int64_t n4 = n / 4;
+	const int tail = (int)(n - n4 * 4);
+	const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n4 + 255) / 256, (int64_t)num_cu * 8));
+	hipLaunchKernelGGL(copy_field_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+					   reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4, src + n4 * 4,
+					   dst + n4 * 4, tail);
+	return hipGetLastError();
 }
 
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream)

@@ -1,6 +1,7 @@
 // The microenvironment a solver reads (reference: reactions-diffusion/biofvm/include/biofvm/microenvironment.h:20-82).
 // The public fields are the data contract of every solver backend.  XML
-// configuration and VTK output are not part of this backend.
+// configuration is not part of this backend; VTK output is (vtk_serializer.h),
+// attached by the caller.
 #pragma once
 
 #include <array>
@@ -14,6 +15,7 @@
 #include "agent_container.h"
 #include "bulk_functor.h"
 #include "mesh.h"
+#include "serializer.h"
 #include "solver.h"
 
 namespace physicore::biofvm {
@@ -42,6 +44,7 @@ public:
 
 	container_ptr agents;
 	solver_ptr solver;
+	serializer_ptr serializer; // called by serialize_state (reference: microenvironment.h:48)
 
 	std::string name, time_units, space_units;
 	real_t diffusion_timestep;

@@ -66,6 +66,10 @@ public:
 
 	// the underlying context, for callers that step or time it through the C ABI directly
 	bfvm_ctx* context() const { return ctx; }
+
+	// the context with writes made through the real_t& accessor uploaded, for
+	// device snapshots of the field (null before initialize)
+	bfvm_ctx* snapshot_context();
 };
 
 } // namespace physicore::biofvm::kernels::hip_solver

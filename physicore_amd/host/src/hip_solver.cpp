@@ -204,6 +204,13 @@ void hip_solver::solve(microenvironment& m, index_t iterations)
 	mirror_valid = false;
 }
 
+bfvm_ctx* hip_solver::snapshot_context()
+{
+	if (ctx && mirror_dirty)
+		push_mirror();
+	return ctx;
+}
+
 real_t hip_solver::get_substrate_density(index_t s, index_t x, index_t y, index_t z) const
 {
 	sync_mirror();

@@ -17,8 +17,13 @@ microenvironment::microenvironment(const cartesian_mesh& mesh, index_t substrate
 // reference: reactions-diffusion/biofvm/src/microenvironment.cpp:87
 void microenvironment::run_single_timestep() { solver->solve(*this, 1); }
 
-// VTK output is not part of this backend
-void microenvironment::serialize_state(real_t) {}
+// reference: reactions-diffusion/biofvm/src/microenvironment.cpp:89-95 (the agents
+// serializer is not part of this backend)
+void microenvironment::serialize_state(real_t current_time)
+{
+	if (serializer)
+		serializer->serialize(*this, current_time);
+}
 
 real_t microenvironment::get_substrate_density(index_t s, index_t x, index_t y, index_t z) const
 {

@@ -11,13 +11,22 @@
 // backend's tests do.  The field is FP32 on the device, so expected values are
 // held to max(reference tolerance, 3e-7 relative).
 //
+// The VTK output tests port reactions-diffusion/biofvm/tests/test_vtk_serializer.cpp;
+// the files are read back with the small reader below instead of the VTK library.
+//
 // Usage: test_hip_solver [--host-only] [filter-substring]
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <functional>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <span>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,6 +35,7 @@
 #include <biofvm/microenvironment.h>
 #include <biofvm/microenvironment_builder.h>
 #include <biofvm/solver_registry.h>
+#include <biofvm/vtk_serializer.h>
 #include <hip_solver/hip_solver.h>
 
 using namespace physicore;
@@ -904,6 +914,380 @@ TEST_HOST(AgentContainerTest, SwapRemove)
 	EXPECT_EQ(moved->secretion_rates()[1], 7.0);
 	EXPECT_EQ(moved->position()[2], 5.0);
 	EXPECT_EQ(m.agents->get_agent_at(1)->volume(), 2.0);
+}
+
+// ---------------------------------------------------------------- VTK output
+
+namespace {
+struct vti_file
+{
+	bool ok = false;
+	std::array<long long, 6> extent {};
+	std::array<double, 3> spacing {};
+	std::vector<std::string> names, types;
+	std::map<std::string, std::vector<double>> arrays;
+};
+
+std::string vti_attr(const std::string& tag, const std::string& key)
+{
+	const auto k = tag.find(" " + key + "=\"");
+	if (k == std::string::npos)
+		return {};
+	const auto b = k + key.size() + 3;
+	return tag.substr(b, tag.find('"', b) - b);
+}
+
+// ImageData with raw appended Float64 cell arrays and UInt64 block headers
+vti_file read_vti(const std::filesystem::path& path)
+{
+	vti_file r;
+	std::ifstream in(path, std::ios::binary);
+	const std::string s((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+	const auto img = s.find("<ImageData ");
+	const auto app = s.find("<AppendedData encoding=\"raw\">");
+	if (img == std::string::npos || app == std::string::npos || s.find("header_type=\"UInt64\"") == std::string::npos)
+		return r;
+	const std::string itag = s.substr(img, s.find('>', img) - img);
+	std::istringstream ex(vti_attr(itag, "WholeExtent"));
+	for (auto& e : r.extent)
+		ex >> e;
+	std::istringstream sp(vti_attr(itag, "Spacing"));
+	for (auto& e : r.spacing)
+		sp >> e;
+	const auto base = s.find('_', app) + 1;
+	for (auto pos = s.find("<DataArray ", img); pos != std::string::npos && pos < app;
+		 pos = s.find("<DataArray ", pos + 1))
+	{
+		const std::string tag = s.substr(pos, s.find('>', pos) - pos);
+		const std::string name = vti_attr(tag, "Name");
+		r.names.push_back(name);
+		r.types.push_back(vti_attr(tag, "type"));
+		const std::size_t off = base + std::stoull(vti_attr(tag, "offset"));
+		std::uint64_t nbytes = 0;
+		if (off + 8 > s.size())
+			return r;
+		std::memcpy(&nbytes, s.data() + off, 8);
+		if (off + 8 + nbytes > s.size() || nbytes % 8)
+			return r;
+		std::vector<double> v(nbytes / 8);
+		std::memcpy(v.data(), s.data() + off + 8, nbytes);
+		r.arrays[name] = std::move(v);
+	}
+	r.ok = true;
+	return r;
+}
+
+std::filesystem::path fresh_dir(const char* name)
+{
+	auto d = std::filesystem::temp_directory_path() / name;
+	std::filesystem::remove_all(d);
+	std::filesystem::create_directories(d);
+	return d;
+}
+
+std::string slurp(const std::filesystem::path& p)
+{
+	std::ifstream f(p);
+	return std::string((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+// create_test_microenvironment, test_vtk_serializer.cpp:44-70
+std::unique_ptr<microenvironment> vtk_env(index_t dims = 3, std::array<index_t, 3> grid = { 3, 3, 3 },
+										  std::array<index_t, 3> voxel = { 20, 20, 20 })
+{
+	microenvironment_builder builder;
+	builder.set_name("test_env");
+	builder.set_time_units("min");
+	builder.set_space_units("um");
+	builder.set_time_step(0.01);
+	builder.resize(dims, { 0, 0, 0 },
+				   { (sindex_t)(grid[0] * voxel[0]), (sindex_t)(grid[1] * voxel[1]), (sindex_t)(grid[2] * voxel[2]) },
+				   voxel);
+	builder.add_density("O2", "mmHg", 1.0, 0.01, 38.0);
+	builder.add_density("Glucose", "mM", 0.5, 0.02, 5.5);
+	builder.select_solver("hip_solver");
+	auto m = builder.build();
+	m->solver->solve(*m, 1);
+	return m;
+}
+
+// a solver whose densities are a function of (s, x, y, z): the synchronous read path
+class pattern_solver : public solver
+{
+public:
+	void initialize(microenvironment&) override {}
+	void solve(microenvironment&, index_t) override {}
+	real_t get_substrate_density(index_t s, index_t x, index_t y, index_t z) const override
+	{
+		return 1000.0 * s + 100.0 * z + 10.0 * y + x + 0.25;
+	}
+	real_t& get_substrate_density(index_t, index_t, index_t, index_t) override
+	{
+		static real_t dummy = 0;
+		return dummy;
+	}
+	void reinitialize_dirichlet(microenvironment&) override {}
+	void recompute_positional_data(microenvironment&) override {}
+};
+} // namespace
+
+TEST_HOST(VtkSerializerTest, ReadPathOfOtherSolvers)
+{
+	const auto dir = fresh_dir("bfvm_vtk_host");
+	microenvironment m(cartesian_mesh(3, { 40, 0, -20 }, { 120, 60, 40 }, { 20, 20, 20 }), 2, 0.01);
+	m.substrates_names = { "A", "B" };
+	m.solver = std::make_unique<pattern_solver>();
+	{
+		vtk_serializer ser(dir.string(), m);
+		EXPECT_TRUE(std::filesystem::exists(dir / "vtk_microenvironment"));
+		ser.serialize(m, 0.5);
+		ser.serialize(m, 1.5);
+	}
+	const auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000001.vti");
+	EXPECT_TRUE(f.ok);
+	const std::array<long long, 6> want_ext = { 2, 6, 0, 3, -1, 2 };
+	for (int k = 0; k < 6; k++)
+		EXPECT_EQ(f.extent[k], want_ext[k]);
+	EXPECT_EQ(f.spacing[2], 20.0);
+	EXPECT_EQ(f.names.size(), (std::size_t)2);
+	EXPECT_EQ(f.types[0], std::string("Float64"));
+	const auto& a = f.arrays.at("B");
+	EXPECT_EQ(a.size(), m.mesh.voxel_count());
+	for (index_t z = 0; z < 3; z++)
+		for (index_t y = 0; y < 3; y++)
+			for (index_t x = 0; x < 4; x++)
+				EXPECT_EQ(a[m.mesh.linearize(x, y, z)], 1000.0 + 100.0 * z + 10.0 * y + x + 0.25);
+	// vtk_serializer_base.cpp:22-37
+	const std::string want_pvd = "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\" "
+								 "byte_order=\"LittleEndian\">\n  <Collection>\n"
+								 "    <DataSet timestep=\"0.500000\" group=\"\" part=\"0\" "
+								 "file=\"vtk_microenvironment/microenvironment_000000.vti\" />\n"
+								 "    <DataSet timestep=\"1.500000\" group=\"\" part=\"0\" "
+								 "file=\"vtk_microenvironment/microenvironment_000001.vti\" />\n"
+								 "  </Collection>\n</VTKFile>";
+	EXPECT_TRUE(slurp(dir / "microenvironment.pvd") == want_pvd);
+	std::filesystem::remove_all(dir);
+}
+
+// test_vtk_serializer.cpp:74-160
+TEST_GPU(VtkSerializerTest, ConstructorFilesAndPvd)
+{
+	const auto dir = fresh_dir("bfvm_vtk_files");
+	auto m = vtk_env();
+	vtk_serializer ser(dir.string(), *m);
+	EXPECT_TRUE(std::filesystem::exists(dir / "vtk_microenvironment"));
+	ser.serialize(*m, 0.1);
+	ser.serialize(*m, 0.2);
+	ser.serialize(*m, 0.0);
+	ser.flush();
+	for (const char* n : { "microenvironment_000000.vti", "microenvironment_000001.vti", "microenvironment_000002.vti" })
+		EXPECT_TRUE(std::filesystem::exists(dir / "vtk_microenvironment" / n));
+	const std::string pvd = slurp(dir / "microenvironment.pvd");
+	for (const char* frag : { "<?xml version=\"1.0\"?>", "<VTKFile type=\"Collection\"", "<Collection>", "</Collection>",
+							  "</VTKFile>", "timestep=\"0.1", "timestep=\"0.2", "microenvironment_000000.vti",
+							  "microenvironment_000001.vti" })
+		EXPECT_TRUE(pvd.find(frag) != std::string::npos);
+	std::filesystem::remove_all(dir);
+}
+
+// test_vtk_serializer.cpp:162-205, :322-345
+TEST_GPU(VtkSerializerTest, VtkFileStructure)
+{
+	const auto dir = fresh_dir("bfvm_vtk_structure");
+	auto m = vtk_env();
+	vtk_serializer ser(dir.string(), *m);
+	ser.serialize(*m, 0.0);
+	ser.flush();
+	const auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+	EXPECT_TRUE(f.ok);
+	for (int a = 0; a < 3; a++)
+	{
+		EXPECT_EQ(f.extent[2 * a + 1] - f.extent[2 * a] + 1, 4LL); // points: grid_shape + 1
+		EXPECT_EQ(f.spacing[a], 20.0);
+	}
+	EXPECT_EQ(f.arrays.count("O2"), (std::size_t)1);
+	EXPECT_EQ(f.arrays.count("Glucose"), (std::size_t)1);
+	EXPECT_EQ(f.arrays.at("O2").size(), (std::size_t)27);
+	EXPECT_EQ(f.arrays.at("Glucose").size(), (std::size_t)27);
+	for (const auto& t : f.types)
+		EXPECT_EQ(t, std::string("Float64")); // real_t = double
+	std::filesystem::remove_all(dir);
+}
+
+// test_vtk_serializer.cpp:207-227
+TEST_GPU(VtkSerializerTest, HandleDifferentMeshDimensions)
+{
+	const auto dir = fresh_dir("bfvm_vtk_dims");
+	{
+		auto m1 = vtk_env(1, { 5, 1, 1 }, { 10, 1, 1 });
+		vtk_serializer ser(dir.string(), *m1);
+		ser.serialize(*m1, 0.0);
+		ser.flush();
+		const auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+		EXPECT_TRUE(f.ok);
+		EXPECT_EQ(f.extent[1], 5LL);
+		EXPECT_EQ(f.extent[5], 0LL);
+		EXPECT_EQ(f.spacing[2], 0.0);
+		EXPECT_EQ(f.arrays.at("O2").size(), (std::size_t)5);
+	}
+	{
+		auto m2 = vtk_env(2, { 4, 4, 1 }, { 15, 15, 1 });
+		vtk_serializer ser(dir.string(), *m2);
+		ser.serialize(*m2, 0.0);
+		ser.flush();
+		const auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+		EXPECT_TRUE(f.ok);
+		EXPECT_EQ(f.extent[3], 4LL);
+		EXPECT_EQ(f.spacing[0], 15.0);
+		EXPECT_EQ(f.arrays.at("Glucose").size(), (std::size_t)16);
+	}
+	std::filesystem::remove_all(dir);
+}
+
+// test_vtk_serializer.cpp:229-320
+TEST_GPU(VtkSerializerTest, SingleAndManySubstrates)
+{
+	const auto dir = fresh_dir("bfvm_vtk_many");
+	const std::vector<std::string> names = { "O2", "Glucose", "Lactate", "ATP", "CO2", "H2O" };
+	microenvironment_builder builder;
+	builder.set_name("many_substrates_env");
+	builder.resize(3, { 0, 0, 0 }, { 40, 40, 40 }, { 20, 20, 20 });
+	for (std::size_t i = 0; i < names.size(); ++i)
+	{
+		builder.add_density(names[i], "unit", 1.0, 0.01, static_cast<real_t>(10 + i));
+		builder.add_boundary_dirichlet_conditions(
+			i, { (real_t)(15 + i), (real_t)(14 + i), (real_t)(13 + i) },
+			{ (real_t)(20 + i), (real_t)(19 + i), (real_t)(18 + i) }, { true, true, true }, { true, true, true });
+	}
+	builder.select_solver("hip_solver");
+	auto m = builder.build();
+	m->solver->initialize(*m);
+	vtk_serializer ser(dir.string(), *m);
+	ser.serialize(*m, 0.0);
+	ser.flush();
+	const auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+	EXPECT_TRUE(f.ok);
+	EXPECT_EQ(f.names.size(), names.size());
+	for (std::size_t i = 0; i < names.size(); ++i)
+	{
+		EXPECT_EQ(f.arrays.count(names[i]), (std::size_t)1);
+		EXPECT_EQ(f.arrays.at(names[i])[0], (double)(10 + i)); // initial condition, before any step
+	}
+	std::filesystem::remove_all(dir);
+}
+
+// test_vtk_serializer.cpp:322-357
+TEST_GPU(VtkSerializerTest, NonZeroBoundingBoxMins)
+{
+	const auto dir = fresh_dir("bfvm_vtk_offset");
+	microenvironment_builder builder;
+	builder.set_name("offset_env");
+	builder.resize(3, { 100, 200, 300 }, { 160, 260, 360 }, { 20, 20, 20 });
+	builder.add_density("O2", "mmHg", 1.0, 0.01, 35.0);
+	builder.add_boundary_dirichlet_conditions(0, { 38.0, 36.0, 34.0 }, { 42.0, 40.0, 38.0 }, { true, true, true },
+											  { true, true, true });
+	builder.select_solver("hip_solver");
+	auto m = builder.build();
+	m->solver->initialize(*m);
+	vtk_serializer ser(dir.string(), *m);
+	ser.serialize(*m, 0.0);
+	ser.flush();
+	const auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+	const std::array<long long, 6> want = { 5, 8, 10, 13, 15, 18 };
+	for (int k = 0; k < 6; k++)
+		EXPECT_EQ(f.extent[k], want[k]);
+	std::filesystem::remove_all(dir);
+}
+
+// test_vtk_serializer.cpp:384-476
+TEST_GPU(VtkSerializerTest, BoundaryConditionsEffect)
+{
+	const auto dir = fresh_dir("bfvm_vtk_boundary");
+	microenvironment_builder builder;
+	builder.set_name("boundary_test_env");
+	builder.resize(3, { 0, 0, 0 }, { 60, 60, 60 }, { 20, 20, 20 });
+	builder.add_density("O2", "mmHg", 0, 0, 20.0);
+	builder.add_density("Glucose", "mM", 0.5, 0.02, 5.5);
+	builder.add_boundary_dirichlet_conditions(0, { 100.0, 20.0, 20.0 }, { 5.0, 20.0, 20.0 }, { true, false, false },
+											  { true, false, false });
+	builder.select_solver("hip_solver");
+	auto m = builder.build();
+	m->solver->solve(*m, 1);
+	vtk_serializer ser(dir.string(), *m);
+	ser.serialize(*m, 0.0);
+	ser.flush();
+	auto f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+	EXPECT_TRUE(f.ok);
+	EXPECT_EQ(f.arrays.at("O2").size(), m->mesh.voxel_count());
+	for (index_t z = 0; z < 3; ++z)
+		for (index_t y = 0; y < 3; ++y)
+			for (index_t x = 0; x < 3; ++x)
+			{
+				const std::size_t v = m->mesh.linearize(x, y, z);
+				EXPECT_EQ(f.arrays.at("O2")[v], x == 0 ? 100.0 : (x == 2 ? 5.0 : 20.0));
+				EXPECT_EQ(f.arrays.at("Glucose")[v], m->get_substrate_density(1, x, y, z));
+			}
+	m->solver->solve(*m, 1);
+	ser.serialize(*m, 0.1);
+	ser.flush();
+	f = read_vti(dir / "vtk_microenvironment" / "microenvironment_000001.vti");
+	EXPECT_TRUE(f.ok);
+	for (index_t z = 0; z < 3; ++z)
+		for (index_t y = 0; y < 3; ++y)
+			for (index_t x = 0; x < 3; ++x)
+				EXPECT_EQ(f.arrays.at("Glucose")[m->mesh.linearize(x, y, z)], m->get_substrate_density(1, x, y, z));
+	std::filesystem::remove_all(dir);
+}
+
+// no counterpart in the reference: snapshots taken while stepping goes on hold
+// the field of the step they were taken at, and writes through the real_t&
+// accessor reach them
+TEST_GPU(VtkSerializerTest, SnapshotsWhileStepping)
+{
+	const auto dir = fresh_dir("bfvm_vtk_async");
+	auto make = [] {
+		microenvironment_builder builder;
+		builder.set_name("async_env");
+		builder.resize(3, { 0, 0, 0 }, { 800, 640, 480 }, { 20, 20, 20 });
+		builder.add_density("A", "u", 500.0, 0.1, 1.0);
+		builder.add_density("B", "u", 20.0, 0.5, 2.0);
+		builder.add_boundary_dirichlet_conditions(0, { 10.0, 0, 0 }, { 0, 0, 0 }, { true, false, false },
+												  { false, false, false });
+		builder.select_solver("hip_solver");
+		return builder.build();
+	};
+	auto m = make();
+	auto twin = make();
+	vtk_serializer ser(dir.string(), *m);
+	m->solver->solve(*m, 3);
+	m->solver->get_substrate_density(1, 5, 6, 7) = 42.0; // host write before the snapshot
+	ser.serialize(*m, 0.03);
+	m->solver->solve(*m, 7); // queued behind the snapshot, runs during its host copy
+	ser.serialize(*m, 0.10);
+	m->solver->solve(*m, 5);
+	ser.flush();
+	twin->solver->solve(*twin, 3);
+	twin->solver->get_substrate_density(1, 5, 6, 7) = 42.0;
+	auto f0 = read_vti(dir / "vtk_microenvironment" / "microenvironment_000000.vti");
+	EXPECT_TRUE(f0.ok);
+	std::size_t mismatches = 0;
+	const auto& g = twin->mesh.grid_shape;
+	for (index_t z = 0; z < g[2]; ++z)
+		for (index_t y = 0; y < g[1]; ++y)
+			for (index_t x = 0; x < g[0]; ++x)
+				for (index_t s = 0; s < 2; ++s)
+					mismatches += f0.arrays.at(s ? "B" : "A")[twin->mesh.linearize(x, y, z)] !=
+								  twin->get_substrate_density(s, x, y, z);
+	EXPECT_EQ(mismatches, (std::size_t)0);
+	twin->solver->solve(*twin, 7);
+	auto f1 = read_vti(dir / "vtk_microenvironment" / "microenvironment_000001.vti");
+	mismatches = 0;
+	for (index_t z = 0; z < g[2]; ++z)
+		for (index_t y = 0; y < g[1]; ++y)
+			for (index_t x = 0; x < g[0]; ++x)
+				mismatches += f1.arrays.at("A")[twin->mesh.linearize(x, y, z)] != twin->get_substrate_density(0, x, y, z);
+	EXPECT_EQ(mismatches, (std::size_t)0);
+	std::filesystem::remove_all(dir);
 }
 
 // ---------------------------------------------------------------- main
