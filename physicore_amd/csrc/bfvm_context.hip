@@ -389,11 +389,12 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 		return;
 	}
 	const int outer = axis == 1 ? nz : ny;
-	if (variant == kSweepReg)
+	if (variant == kSweepReg || variant == kSweepTwist)
 	{
 		// flattened (outer, column) space: x columns are the S lines of a row
 		const int64_t total = axis == 0 ? (int64_t)ny * nz * S : (int64_t)outer * L;
-		for (int64_t f0 = 0; f0 < total; f0 += kColsPerTile)
+		const int per_tile = variant == kSweepTwist ? kTwistColsPerTile : kColsPerTile;
+		for (int64_t f0 = 0; f0 < total; f0 += per_tile)
 			t.push_back({ dom, (int)f0, 0, 0 });
 		return;
 	}
@@ -450,10 +451,18 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepReg;
 				c->nmax[axis] = bucket;
 			}
-			// register kernels pad the factor rows to their bucket
+			// y/z lines of 257..512: two lanes per line in registers
+			else if (axis > 0 && c->fp64 && twist_bucket(n))
+			{
+				c->variant[axis] = kSweepTwist;
+				c->nmax[axis] = twist_bucket(n);
+			}
+			// register kernels pad the factor rows to their bucket (the twisted
+			// ones also carry a 16-entry zero prefix per substrate)
 			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows ||
 								c->variant[axis] == kSweepPipe;
-			c->fac_cap[axis] = (padded ? c->nmax[axis] : n) * c->S;
+			c->fac_cap[axis] = c->variant[axis] == kSweepTwist ? (c->nmax[axis] + 16) * c->S
+															   : (padded ? c->nmax[axis] : n) * c->S;
 			if (c->variant[axis] == kSweepPipe)
 			{
 				domain_pipe_tiles(axis, 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, c->pipe[axis], t);

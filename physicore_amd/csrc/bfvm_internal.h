@@ -203,7 +203,8 @@ enum SweepVariant
 	kSweepReg = 1,	// one line per lane in VGPRs, read straight from HBM (x: stride S)
 	kSweepScan = 2, // x only: lanes along the row, affine scans; nmax carries voxels per lane
 	kSweepRows = 3, // x only: rows staged in LDS, one line per lane in VGPRs
-	kSweepPipe = 4	// register lines fed by buffer-to-LDS prefetch of the next tile (FP64)
+	kSweepPipe = 4,	// register lines fed by buffer-to-LDS prefetch of the next tile (FP64)
+	kSweepTwist = 5 // y/z lines of 257..512: two lanes per line, twisted elimination (FP64)
 };
 
 // Geometry of the pipelined sweeps (kSweepPipe): the per-wave tile buffer in
@@ -233,7 +234,11 @@ __host__ inline int xscan_vpl(int nx, int S)
 }
 constexpr int kRegMaxN = 256;
 constexpr int kColsPerTile = kSweepThreads; // y/z register tiles: 256 flattened columns
-int reg_bucket(int n);						// 16..256, or 0 when n > kRegMaxN
+int reg_bucket(int n);
+// Half-line bucket of the twisted kernels (lines of kRegMaxN+1 .. 2*kRegMaxN):
+// ceil(n/2) rounded up to 16, or 0 outside that range.
+int twist_bucket(int n);
+constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column						// 16..256, or 0 when n > kRegMaxN
 
 // LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);

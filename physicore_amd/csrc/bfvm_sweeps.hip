@@ -18,6 +18,10 @@ kernel_fn pick_cols_x32(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
+kernel_fn pick_twist_y_a(int nmax, bool faces);
+kernel_fn pick_twist_y_b(int nmax, bool faces);
+kernel_fn pick_twist_z_a(int nmax, bool faces);
+kernel_fn pick_twist_z_b(int nmax, bool faces);
 kernel_fn pick_xscan(int S, int vpl, bool fp64);
 pipe_fn pick_pipe_x_s1(int nmax, bool faces);
 pipe_fn pick_pipe_x_s2(int nmax, bool faces);
@@ -92,6 +96,15 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		return fp64 ? pick_lds<double>(axis) : pick_lds<float>(axis);
 	if (variant == kSweepRows)
 		return axis == 0 ? pick_rows(nmax, fp64) : nullptr;
+	if (variant == kSweepTwist)
+	{
+		if (axis == 0 || !fp64)
+			return nullptr;
+		const bool lo = nmax <= 192;
+		if (axis == 1)
+			return lo ? pick_twist_y_a(nmax, faces) : pick_twist_y_b(nmax, faces);
+		return lo ? pick_twist_z_a(nmax, faces) : pick_twist_z_b(nmax, faces);
+	}
 	// FP64 instantiations split at 128 lines (two translation units per axis)
 	const bool lo = nmax <= 128;
 	if (axis == 0)
@@ -108,6 +121,13 @@ int reg_bucket(int n)
 	if (n > kRegMaxN)
 		return 0;
 	return ((n + 15) / 16) * 16;
+}
+
+int twist_bucket(int n)
+{
+	if (n <= kRegMaxN || n > 2 * kRegMaxN)
+		return 0;
+	return (((n + 1) / 2 + 15) / 16) * 16;
 }
 
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
@@ -137,6 +157,18 @@ hipError_t init_sweep_kernels()
 					variant = kSweepRows;
 				}
 				sweep::kernel_fn k = sweep::pick(axis, variant, nmax, p & 1, p & 2, 1);
+				if (!k)
+					return hipErrorInvalidValue;
+				hipError_t e =
+					hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+				if (e != hipSuccess)
+					return e;
+			}
+	for (int axis = 1; axis < 3; axis++)
+		for (int nmax = twist_bucket(kRegMaxN + 1); nmax <= kRegMaxN; nmax += 16)
+			for (int faces = 0; faces < 2; faces++)
+			{
+				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, faces, 1);
 				if (!k)
 					return hipErrorInvalidValue;
 				hipError_t e =

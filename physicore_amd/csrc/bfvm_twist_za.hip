@@ -1,0 +1,26 @@
+// Instantiations of the twisted two-lane z-sweep, lines of 257..384 voxels
+// (half-line buckets of 16).
+#include "bfvm_twist_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_twist_z_a(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_twist_kernel<2, N, true> : sweep_twist_kernel<2, N, false>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(144)
+		BFVM_BUCKET(160)
+		BFVM_BUCKET(176)
+		BFVM_BUCKET(192)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

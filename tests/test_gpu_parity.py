@@ -55,6 +55,10 @@ SHAPES = [
     (1, (101, 1, 1), 2), (3, (129, 3, 4), 1), (3, (3, 130, 5), 2), (3, (4, 5, 200), 4),
     (3, (200, 6, 5), 4), (3, (257, 3, 2), 2), (3, (300, 4, 3), 1), (3, (77, 5, 4), 6),
     (3, (6, 300, 3), 2), (3, (5, 4, 280), 3), (3, (130, 7, 3), 8), (1, (600, 1, 1), 3),
+    # y/z lines of 257..512 (two lanes per line): odd and even lengths, both
+    # halves' bucket edges, the longest line
+    (3, (5, 257, 3), 2), (3, (3, 401, 4), 3), (3, (4, 3, 512), 1), (3, (7, 5, 385), 2),
+    (2, (9, 333, 1), 2),
 ]
 
 
@@ -211,6 +215,26 @@ def test_dirichlet_precedence_interior_over_faces_random():
             spec.set_face(a, side, rng.uniform(0, 9, S), rng.integers(0, 2, S))
     vox = [(0, 0, 0), (8, 7, 6), (4, 4, 4), (0, 3, 6), (4, 4, 4)]
     spec.set_interior(vox, rng.uniform(10, 20, (5, S)), rng.integers(0, 2, (5, S)))
+    f0 = rng.uniform(0, 1, p.shape)
+    s = orc.Solver(p, f0.astype(np.float32).astype(np.float64), dirichlet=spec)
+    s.solve(3)
+    d = gpu_domain(p, field=f0, dirichlet=spec)
+    d.step(3)
+    assert rel_linf(d.field(), s.field).max() < 3e-6
+
+
+@pytest.mark.parametrize("dims,grid", [(3, (6, 5, 301)), (2, (7, 333, 1)), (3, (4, 290, 3))])
+def test_long_lines_with_faces(dims, grid):
+    """Lines of 257..512 voxels (two lanes per line) in the sweep that applies
+    the Dirichlet faces (z in 3-D, y in 2-D), every face set for some
+    substrate, and lines lying in faces of the other axes."""
+    rng = np.random.default_rng(sum(grid))
+    S = 2
+    p = orc.Problem(dims, grid, (10, 10, 10), 0.01, [800.0, 3.0], [0.1, 0.6])
+    spec = orc.DirichletSpec(S, dims)
+    for a in range(dims):
+        for side in ("min", "max"):
+            spec.set_face(a, side, rng.uniform(0, 9, S), rng.integers(0, 2, S))
     f0 = rng.uniform(0, 1, p.shape)
     s = orc.Solver(p, f0.astype(np.float32).astype(np.float64), dirichlet=spec)
     s.solve(3)
