@@ -236,7 +236,7 @@ constexpr int kRegMaxN = 256;
 constexpr int kColsPerTile = kSweepThreads; // y/z register tiles: 256 flattened columns
 int reg_bucket(int n);
 // Half-line bucket of the twisted kernels (lines of kRegMaxN+1 .. 2*kRegMaxN):
-// ceil(n/2) rounded up to 16, or 0 outside that range.
+// ceil(n/2) rounded up to 8, or 0 outside that range.
 int twist_bucket(int n);
 constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column						// 16..256, or 0 when n > kRegMaxN
 

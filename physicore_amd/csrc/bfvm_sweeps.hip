@@ -3,6 +3,7 @@
 // kernels are instantiated in bfvm_sweeps_{y,z}{64,32}.hip and bfvm_sweeps_rows.hip
 // (separate translation units so they compile in parallel).
 #include "bfvm_pipe_kernels.h"
+#include "bfvm_twist_kernels.h"
 
 #include <algorithm>
 
@@ -20,8 +21,12 @@ kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
 kernel_fn pick_twist_y_a(int nmax, bool faces);
 kernel_fn pick_twist_y_b(int nmax, bool faces);
+kernel_fn pick_twist_y_c(int nmax, bool faces);
+kernel_fn pick_twist_y_d(int nmax, bool faces);
 kernel_fn pick_twist_z_a(int nmax, bool faces);
 kernel_fn pick_twist_z_b(int nmax, bool faces);
+kernel_fn pick_twist_z_c(int nmax, bool faces);
+kernel_fn pick_twist_z_d(int nmax, bool faces);
 kernel_fn pick_xscan(int S, int vpl, bool fp64);
 pipe_fn pick_pipe_x_s1(int nmax, bool faces);
 pipe_fn pick_pipe_x_s2(int nmax, bool faces);
@@ -100,10 +105,11 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 	{
 		if (axis == 0 || !fp64)
 			return nullptr;
-		const bool lo = nmax <= 192;
-		if (axis == 1)
-			return lo ? pick_twist_y_a(nmax, faces) : pick_twist_y_b(nmax, faces);
-		return lo ? pick_twist_z_a(nmax, faces) : pick_twist_z_b(nmax, faces);
+		// instantiation groups of four buckets: 136..160, 168..192, 200..224, 232..256
+		const int g = nmax <= 160 ? 0 : nmax <= 192 ? 1 : nmax <= 224 ? 2 : 3;
+		static kernel_fn (*const y[4])(int, bool) = { pick_twist_y_a, pick_twist_y_b, pick_twist_y_c, pick_twist_y_d };
+		static kernel_fn (*const z[4])(int, bool) = { pick_twist_z_a, pick_twist_z_b, pick_twist_z_c, pick_twist_z_d };
+		return axis == 1 ? y[g](nmax, faces) : z[g](nmax, faces);
 	}
 	// FP64 instantiations split at 128 lines (two translation units per axis)
 	const bool lo = nmax <= 128;
@@ -127,7 +133,7 @@ int twist_bucket(int n)
 {
 	if (n <= kRegMaxN || n > 2 * kRegMaxN)
 		return 0;
-	return (((n + 1) / 2 + 15) / 16) * 16;
+	return (((n + 1) / 2 + 7) / 8) * 8;
 }
 
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
@@ -139,6 +145,8 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 		return fac + (size_t)std::max(1, x_rows_per_tile(S, L, fac_cap)) * x_row_pitch(L, S) * sizeof(float);
 	if (variant == kSweepLds)
 		return fac + (size_t)n * kLinesPerTile * sizeof(float);
+	if (variant == kSweepTwist)
+		return fac + (size_t)sweep::kTwistExchange * sizeof(double);
 	return fac;
 }
 
@@ -165,7 +173,7 @@ hipError_t init_sweep_kernels()
 					return e;
 			}
 	for (int axis = 1; axis < 3; axis++)
-		for (int nmax = twist_bucket(kRegMaxN + 1); nmax <= kRegMaxN; nmax += 16)
+		for (int nmax = twist_bucket(kRegMaxN + 1); nmax <= kRegMaxN; nmax += 8)
 			for (int faces = 0; faces < 2; faces++)
 			{
 				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, faces, 1);

@@ -1,0 +1,26 @@
+// Instantiations of the twisted two-lane y-sweep, lines of 385..448 voxels
+// (half-line buckets of 8).
+#include "bfvm_twist_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_twist_y_c(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_twist_kernel<1, N, true> : sweep_twist_kernel<1, N, false>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(200)
+		BFVM_BUCKET(208)
+		BFVM_BUCKET(216)
+		BFVM_BUCKET(224)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

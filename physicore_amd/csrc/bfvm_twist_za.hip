@@ -1,5 +1,5 @@
-// Instantiations of the twisted two-lane z-sweep, lines of 257..384 voxels
-// (half-line buckets of 16).
+// Instantiations of the twisted two-lane z-sweep, lines of 257..320 voxels
+// (half-line buckets of 8).
 #include "bfvm_twist_kernels.h"
 
 namespace bfvm {
@@ -12,10 +12,10 @@ kernel_fn pick_twist_z_a(int nmax, bool faces)
 		return faces ? sweep_twist_kernel<2, N, true> : sweep_twist_kernel<2, N, false>;
 	switch (nmax)
 	{
+		BFVM_BUCKET(136)
 		BFVM_BUCKET(144)
+		BFVM_BUCKET(152)
 		BFVM_BUCKET(160)
-		BFVM_BUCKET(176)
-		BFVM_BUCKET(192)
 	default:
 		return nullptr;
 	}

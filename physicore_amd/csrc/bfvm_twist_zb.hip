@@ -1,5 +1,5 @@
-// Instantiations of the twisted two-lane z-sweep, lines of 385..512 voxels
-// (half-line buckets of 16).
+// Instantiations of the twisted two-lane z-sweep, lines of 321..384 voxels
+// (half-line buckets of 8).
 #include "bfvm_twist_kernels.h"
 
 namespace bfvm {
@@ -12,10 +12,10 @@ kernel_fn pick_twist_z_b(int nmax, bool faces)
 		return faces ? sweep_twist_kernel<2, N, true> : sweep_twist_kernel<2, N, false>;
 	switch (nmax)
 	{
-		BFVM_BUCKET(208)
-		BFVM_BUCKET(224)
-		BFVM_BUCKET(240)
-		BFVM_BUCKET(256)
+		BFVM_BUCKET(168)
+		BFVM_BUCKET(176)
+		BFVM_BUCKET(184)
+		BFVM_BUCKET(192)
 	default:
 		return nullptr;
 	}

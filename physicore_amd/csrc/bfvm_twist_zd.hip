@@ -1,0 +1,26 @@
+// Instantiations of the twisted two-lane z-sweep, lines of 449..512 voxels
+// (half-line buckets of 8).
+#include "bfvm_twist_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_twist_z_d(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_twist_kernel<2, N, true> : sweep_twist_kernel<2, N, false>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(232)
+		BFVM_BUCKET(240)
+		BFVM_BUCKET(248)
+		BFVM_BUCKET(256)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm
