@@ -206,16 +206,21 @@ int bfvm_stream(bfvm_ctx* ctx, void** stream);
  * (0 when a stage does not run). */
 int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 
-/* Kernel of the x sweep (lines of up to 256 voxels; longer lines are solved
- * inside LDS tiles whatever is chosen). */
+/* Kernel of the x sweep.  A kernel that does not take the shape falls back to
+ * the strided lines; rows longer than 256 voxels run on the scan kernel where
+ * it fits (up to 512 voxels, S*ceil(nx/64) <= 32), else inside LDS tiles. */
 typedef enum bfvm_x_sweep
 {
 	BFVM_X_AUTO = 0,	  /* the library's default for the shape */
 	BFVM_X_STRIDED = 1,	  /* one line per lane in VGPRs, read from HBM with stride S */
 	BFVM_X_TRANSPOSE = 2, /* rows staged in LDS, one line per lane in VGPRs */
 	BFVM_X_SCAN = 3,	  /* lanes along the row, lane-parallel affine scans */
-	BFVM_X_PIPELINED = 4  /* rows prefetched into LDS while the previous tile is solved
+	BFVM_X_PIPELINED = 4, /* rows prefetched into LDS while the previous tile is solved
 						   * (FP64, S in {1, 2, 4, 8}, nx*S % 4 == 0) */
+	BFVM_X_CHUNKED = 5	  /* one line per lane in VGPRs, filled and emptied through an
+						   * LDS transpose of 32-voxel chunks with 16-byte accesses
+						   * (FP64, S in {4, 8}, rows of up to 200 voxels, at least
+						   * 64/S rows; other shapes use the strided lines) */
 } bfvm_x_sweep;
 
 /* Kernel selection and rounding.  persistent_wgs_per_cu: 0 = the default per

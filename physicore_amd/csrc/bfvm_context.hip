@@ -27,11 +27,20 @@ constexpr int kPipeMinLine = 192;
 // strided / LDS transpose / scan, us): S = 1, nx = 200: 64.6 / 38.4 / 21.1;
 // S = 2, nx = 64: 23.5 / 18.9 / 22.9; S = 2, nx = 200: 58.9 / 40.0 / 25.9;
 // S = 4, nx = 64: 16.0 / 20.3 / 33.2; S = 4, nx = 200: 38.8 / 50.5 / 41.0;
-// S = 8, nx = 200: 63.9 / 111 / 98.6.  200^3 x 4: strided 76.5, scan 80.7.
+// S = 8, nx = 200: 63.9 / 111 / 98.6.  The chunked transpose (S = 4, 8,
+// rows up to 200) reads the strided kernel's lines with 16-byte accesses
+// (strided / scan / chunked, us): 200^3 x 4: 75.9 / 80.5 / 60.8; 200^3 x 8:
+// 151 / 203 / 121; 160^3 x 4: 41.2 / - / 40.2; 100^3 x 8: 37.1 / 39.5 / 28.7;
+// 32 x 256^2 x 4: 23.2 / - / 22.4; 8 x 512^2 x 4: 35.5 / - / 22.6; and slower
+// for 64 x 128^2 x 4: 16.3 / - / 17.3, 64 x 128^2 x 8: 25.6 / - / 31.0,
+// 16 x 256^2 x 8: 23.8 / - / 32.1.  Shapes the chosen kernel does not take
+// fall back to the strided lines.
 int default_x_sweep(int nx, int S)
 {
 	if (S <= 2 || nx > bfvm::kRegMaxN)
 		return (S == 2 && nx < 96) ? BFVM_X_TRANSPOSE : BFVM_X_SCAN;
+	if ((S == 4 || S == 8) && bfvm::xt_bucket(nx) && (nx >= 96 || (S == 4 && nx <= 32)))
+		return BFVM_X_CHUNKED;
 	return BFVM_X_STRIDED;
 }
 } // namespace
@@ -380,7 +389,8 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 			t.push_back({ dom, r0, std::min(kScanRowsPerTile, rows - r0), 0 });
 		return;
 	}
-	if (axis == 0 && variant != kSweepReg)
+	// x tiles of whole rows (LDS-staged kernels): first row, row count
+	if (axis == 0 && variant != kSweepReg && variant != kSweepXT)
 	{
 		const int R = std::max(1, x_rows_per_tile(S, L, fac_cap));
 		const int rows = ny * nz;
@@ -389,7 +399,7 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 		return;
 	}
 	const int outer = axis == 1 ? nz : ny;
-	if (variant == kSweepReg || variant == kSweepTwist)
+	if (variant == kSweepReg || variant == kSweepTwist || variant == kSweepXT)
 	{
 		// flattened (outer, column) space: x columns are the S lines of a row
 		const int64_t total = axis == 0 ? (int64_t)ny * nz * S : (int64_t)outer * L;
@@ -435,7 +445,14 @@ int build_tiles(bfvm_ctx* c)
 				c->nmax[axis] = pbucket;
 				c->pipe[axis] = pg;
 			}
-			// the scan kernel does not apply faces, which 1-D domains need in this sweep
+			// the scan and chunked kernels do not apply faces, which 1-D domains need in this sweep
+			// (the chunked kernel also needs a full wave of rows, 64/S)
+			else if (axis == 0 && xs == BFVM_X_CHUNKED && xt_bucket(n) && c->fp64 && (c->S == 4 || c->S == 8) &&
+					 c->n[1] * c->n[2] >= 64 / c->S && !(c->dims == 1 && c->face_mask))
+			{
+				c->variant[axis] = kSweepXT;
+				c->nmax[axis] = xt_bucket(n);
+			}
 			else if (axis == 0 && xs == BFVM_X_SCAN && vpl && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepScan;
@@ -460,7 +477,7 @@ int build_tiles(bfvm_ctx* c)
 			// register kernels pad the factor rows to their bucket (the twisted
 			// ones also carry a 16-entry zero prefix per substrate)
 			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows ||
-								c->variant[axis] == kSweepPipe;
+								c->variant[axis] == kSweepPipe || c->variant[axis] == kSweepXT;
 			c->fac_cap[axis] = c->variant[axis] == kSweepTwist ? (c->nmax[axis] + 16) * c->S
 															   : (padded ? c->nmax[axis] : n) * c->S;
 			if (c->variant[axis] == kSweepPipe)
@@ -866,7 +883,7 @@ int batch_prepare(bfvm_batch* b)
 				continue;
 			const int v = c->variant[axis];
 			// kernels templated on S: the x scan, the pipelined x kernel
-			const int keyS = (v == kSweepScan || (v == kSweepPipe && axis == 0)) ? c->S : 0;
+			const int keyS = (v == kSweepScan || v == kSweepXT || (v == kSweepPipe && axis == 0)) ? c->S : 0;
 			groups[{ v, c->nmax[axis], c->fp64 ? 1 : 0, keyS }].push_back((int)i);
 		}
 		for (auto& [key, doms] : groups)
@@ -1777,7 +1794,7 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [-1, 64]");
 	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
-	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_PIPELINED)
+	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_CHUNKED)
 		return c->fail(BFVM_ERR_INVALID, "x_sweep must be one of the bfvm_x_sweep values");
 	if (t->pipeline != 0 && t->pipeline != 1)
 		return c->fail(BFVM_ERR_INVALID, "pipeline must be 0 or 1");

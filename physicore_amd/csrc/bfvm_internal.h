@@ -204,7 +204,8 @@ enum SweepVariant
 	kSweepScan = 2, // x only: lanes along the row, affine scans; nmax carries voxels per lane
 	kSweepRows = 3, // x only: rows staged in LDS, one line per lane in VGPRs
 	kSweepPipe = 4,	// register lines fed by buffer-to-LDS prefetch of the next tile (FP64)
-	kSweepTwist = 5 // y/z lines of 257..512: two lanes per line, twisted elimination (FP64)
+	kSweepTwist = 5, // y/z lines of 257..512: two lanes per line, twisted elimination (FP64)
+	kSweepXT = 6	 // x, S in {4, 8}: register lines through a chunked LDS transpose (FP64)
 };
 
 // Geometry of the pipelined sweeps (kSweepPipe): the per-wave tile buffer in
@@ -234,11 +235,18 @@ __host__ inline int xscan_vpl(int nx, int S)
 }
 constexpr int kRegMaxN = 256;
 constexpr int kColsPerTile = kSweepThreads; // y/z register tiles: 256 flattened columns
+// Register bucket: n rounded up to 16 (16..256), or 0 when n > kRegMaxN.
 int reg_bucket(int n);
 // Half-line bucket of the twisted kernels (lines of kRegMaxN+1 .. 2*kRegMaxN):
 // ceil(n/2) rounded up to 8, or 0 outside that range.
 int twist_bucket(int n);
-constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column						// 16..256, or 0 when n > kRegMaxN
+constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column
+// Row bucket of the chunked x kernel: n rounded up to 8, and up to 32 when
+// that leaves a last chunk of 24 voxels (whose rows do not split into 16-byte
+// accesses evenly over a wave); 0 beyond kXtMaxN, where a line, its staging
+// and the recurrence state exceed 256 VGPRs.
+constexpr int kXtMaxN = 200;
+int xt_bucket(int n);
 
 // LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);

@@ -4,6 +4,7 @@
 // (separate translation units so they compile in parallel).
 #include "bfvm_pipe_kernels.h"
 #include "bfvm_twist_kernels.h"
+#include "bfvm_xt_kernels.h"
 
 #include <algorithm>
 
@@ -19,6 +20,10 @@ kernel_fn pick_cols_x32(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
+kernel_fn pick_xt4a(int nmax);
+kernel_fn pick_xt4b(int nmax);
+kernel_fn pick_xt8a(int nmax);
+kernel_fn pick_xt8b(int nmax);
 kernel_fn pick_twist_y_a(int nmax, bool faces);
 kernel_fn pick_twist_y_b(int nmax, bool faces);
 kernel_fn pick_twist_y_c(int nmax, bool faces);
@@ -101,6 +106,13 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		return fp64 ? pick_lds<double>(axis) : pick_lds<float>(axis);
 	if (variant == kSweepRows)
 		return axis == 0 ? pick_rows(nmax, fp64) : nullptr;
+	if (variant == kSweepXT)
+	{
+		if (axis != 0 || !fp64 || (S != 4 && S != 8))
+			return nullptr;
+		const bool lo = nmax <= 96;
+		return S == 4 ? (lo ? pick_xt4a(nmax) : pick_xt4b(nmax)) : (lo ? pick_xt8a(nmax) : pick_xt8b(nmax));
+	}
 	if (variant == kSweepTwist)
 	{
 		if (axis == 0 || !fp64)
@@ -136,6 +148,14 @@ int twist_bucket(int n)
 	return (((n + 1) / 2 + 7) / 8) * 8;
 }
 
+int xt_bucket(int n)
+{
+	if (n < 1 || n > kXtMaxN)
+		return 0;
+	const int b = ((n + 7) / 8) * 8;
+	return b % 32 == 24 ? b + 8 : b;
+}
+
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 {
 	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
@@ -147,6 +167,8 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 		return fac + (size_t)n * kLinesPerTile * sizeof(float);
 	if (variant == kSweepTwist)
 		return fac + (size_t)sweep::kTwistExchange * sizeof(double);
+	if (variant == kSweepXT)
+		return fac + (size_t)(kSweepThreads / 64) * sweep::xt_buffer_floats(S) * sizeof(float);
 	return fac;
 }
 
@@ -172,6 +194,18 @@ hipError_t init_sweep_kernels()
 				if (e != hipSuccess)
 					return e;
 			}
+	for (int S : { 4, 8 })
+		for (int nmax = 8; nmax <= kXtMaxN; nmax += 8)
+		{
+			if (xt_bucket(nmax) != nmax)
+				continue;
+			sweep::kernel_fn k = sweep::pick(0, kSweepXT, nmax, true, false, S);
+			if (!k)
+				return hipErrorInvalidValue;
+			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+			if (e != hipSuccess)
+				return e;
+		}
 	for (int axis = 1; axis < 3; axis++)
 		for (int nmax = twist_bucket(kRegMaxN + 1); nmax <= kRegMaxN; nmax += 8)
 			for (int faces = 0; faces < 2; faces++)

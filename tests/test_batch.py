@@ -46,6 +46,22 @@ def test_batch_equals_alone_bit_exact():
             assert np.array_equal(a.internalized(), m.internalized()), k
 
 
+def test_batch_equals_alone_chunked_x():
+    """Domains on the chunked x kernel (S of 4 or 8) batched with others"""
+    require_gpu()
+    scns = _make(7, 12, extras=True, sides=[8, 16, 23, 32, 40, 57])
+    alone = [_gpu(s) for s in scns]
+    mixed = [_gpu(s) for s in scns]
+    for d in alone + mixed:
+        d.set_tuning(x_sweep=5)
+    b = bfvm.Batch(mixed)
+    for d in alone:
+        d.step(5)
+    b.step(5)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a.field(), m.field()), k
+
+
 def test_batch_graph_equals_eager():
     require_gpu()
     scns = _make(2, 9, extras=True)

@@ -64,11 +64,11 @@ SHAPES = [
 
 # kernel variants: the default (register lines, x kernel per shape), each
 # explicit x kernel (strided register lines, LDS transpose, affine scan,
-# pipelined), pipelined y/z lines, round-to-nearest stores, and every line
-# solved inside an LDS tile
+# pipelined, chunked transpose), pipelined y/z lines, round-to-nearest stores,
+# and every line solved inside an LDS tile
 VARIANTS = {"default": {}, "xstrided": {"x_sweep": 1}, "xrows": {"x_sweep": 2}, "xscan": {"x_sweep": 3},
-            "xpipe": {"x_sweep": 4}, "pipe": {"pipeline": 1}, "nearest": {"round_nearest": 1},
-            "lds": {"force_lds_sweeps": 1}}
+            "xpipe": {"x_sweep": 4}, "xchunk": {"x_sweep": 5}, "pipe": {"pipeline": 1},
+            "nearest": {"round_nearest": 1}, "lds": {"force_lds_sweeps": 1}}
 
 
 @pytest.mark.parametrize("dims,grid,S", SHAPES)
@@ -87,6 +87,35 @@ def test_diffusion_step_matches_oracle(dims, grid, S, quirks, variant):
     d.step(1)
     err = rel_linf(d.field(), want)
     assert err.max() < 2e-6, err
+
+
+# rows for the chunked x kernel: every tail chunk (8, 16, 32 voxels and a
+# bucket rounded up past a 24-voxel tail), row counts that leave a partial last
+# wave (64/S rows per wave), exactly one wave, 2-D, and shapes it does not take
+# (one row, S = 2) where the strided kernel runs instead
+CHUNK_SHAPES = [
+    (3, (200, 13, 3), 4), (3, (184, 9, 2), 8), (3, (161, 5, 4), 4), (3, (8, 17, 3), 8),
+    (3, (40, 16, 1), 4), (2, (97, 23, 1), 4), (3, (59, 7, 3), 8), (3, (128, 8, 1), 8),
+    (3, (192, 31, 2), 4), (1, (150, 1, 1), 4), (3, (60, 10, 10), 2),
+]
+
+
+@pytest.mark.parametrize("dims,grid,S", CHUNK_SHAPES)
+def test_chunked_x_sweep_matches_strided(dims, grid, S):
+    """The chunked x kernel runs the strided kernel's recurrences with its
+    factors and rounding keys: fields identical bit for bit over several steps"""
+    rng = np.random.default_rng(hash((dims, grid, S, 5)) % 2**32)
+    D = 10 ** rng.uniform(0, 5, S)
+    lam = rng.uniform(0.01, 1, S)
+    p = orc.Problem(dims, grid, (20, 20, 20), 0.01, D, lam)
+    f0 = rng.uniform(0, 1, p.shape)
+    a = gpu_domain(p, field=f0)
+    b = gpu_domain(p, field=f0)
+    a.set_tuning(x_sweep=1)
+    b.set_tuning(x_sweep=5)
+    a.step(3)
+    b.step(3)
+    assert np.array_equal(a.field(), b.field())
 
 
 def test_fp32_arithmetic_mode_runs():
