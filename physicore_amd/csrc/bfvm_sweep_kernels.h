@@ -124,6 +124,28 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int voff, in
 // and SGPRs; latency is hidden across the resident waves instead.
 constexpr int kChunk = 16;
 
+// Back substitution of thomas_regs (v holds the rounded forward values).
+template <int NMAX, typename ACC, typename Out>
+__device__ __forceinline__ void thomas_back(float (&v)[NMAX], const double* __restrict__ fb, double c, LineKey ko,
+											Out out)
+{
+	constexpr int nchunks = (NMAX - 1 + kChunk - 1) / kChunk;
+	const ACC cc = (ACC)c;
+	ACC nxt = (ACC)v[NMAX - 1] * (ACC)fb[NMAX - 1];
+	out(NMAX - 1, field_round<ACC>(nxt, ko, weyl29(NMAX - 1)));
+	static_for<nchunks>([&](auto C) {
+		static_for<kChunk>([&](auto K) {
+			constexpr int i = NMAX - 2 - (C * kChunk + K);
+			if constexpr (i >= 0)
+			{
+				nxt = ((ACC)v[i] + cc * nxt) * (ACC)fb[i];
+				out(i, field_round<ACC>(nxt, ko, weyl29(i)));
+			}
+		});
+		__builtin_amdgcn_sched_barrier(0);
+	});
+}
+
 template <int NMAX, typename ACC, typename Out>
 __device__ __forceinline__ void thomas_regs(float (&v)[NMAX], const double* __restrict__ fb,
 											const double* __restrict__ fe, double c, LineKey kf, LineKey ko, Out out)
@@ -141,20 +163,45 @@ __device__ __forceinline__ void thomas_regs(float (&v)[NMAX], const double* __re
 		});
 		__builtin_amdgcn_sched_barrier(0);
 	});
-	const ACC cc = (ACC)c;
-	ACC nxt = (ACC)v[NMAX - 1] * (ACC)fb[NMAX - 1];
-	out(NMAX - 1, field_round<ACC>(nxt, ko, weyl29(NMAX - 1)));
+	thomas_back<NMAX, ACC>(v, fb, c, ko, out);
+}
+
+// thomas_regs with the line's loads interleaved with the forward recurrence:
+// `ld(i)` returns element i.  The first kLoadAhead loads are issued before the
+// recurrence starts, then kChunk more before each chunk of kChunk steps, so the
+// forward steps run as the data arrives instead of after the last load; at
+// most kLoadAhead + kChunk = 56 loads are in flight, within the 63 a wave's
+// vector-memory counter tracks.
+constexpr int kLoadAhead = 40;
+
+template <int NMAX, typename ACC, typename Load, typename Out>
+__device__ __forceinline__ void thomas_regs_streamed(float (&v)[NMAX], Load ld, const double* __restrict__ fb,
+													 const double* __restrict__ fe, double c, LineKey kf, LineKey ko,
+													 Out out)
+{
+	constexpr int nchunks = (NMAX - 1 + kChunk - 1) / kChunk;
+	constexpr int first = NMAX < kLoadAhead ? NMAX : kLoadAhead;
+	static_for<first>([&](auto I) { v[I] = ld((int)I); });
+	ACC prev = 0;
 	static_for<nchunks>([&](auto C) {
 		static_for<kChunk>([&](auto K) {
-			constexpr int i = NMAX - 2 - (C * kChunk + K);
-			if constexpr (i >= 0)
+			constexpr int j = kLoadAhead + C * kChunk + K;
+			if constexpr (j < NMAX)
+				v[j] = ld(j);
+		});
+		if constexpr (C == 0)
+			prev = (ACC)v[0];
+		static_for<kChunk>([&](auto K) {
+			constexpr int i = 1 + C * kChunk + K;
+			if constexpr (i < NMAX)
 			{
-				nxt = ((ACC)v[i] + cc * nxt) * (ACC)fb[i];
-				out(i, field_round<ACC>(nxt, ko, weyl29(i)));
+				prev = (ACC)v[i] + (ACC)fe[i - 1] * prev;
+				v[i] = field_round<ACC>(prev, kf, weyl29(i));
 			}
 		});
 		__builtin_amdgcn_sched_barrier(0);
 	});
+	thomas_back<NMAX, ACC>(v, fb, c, ko, out);
 }
 
 // Face Dirichlet values seen by one line of the y or z sweep.  Faces of the
@@ -273,8 +320,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 		const LineKey ko = line_key(d, 2 * AXIS + 1, (uint32_t)f);
 
 		float v[NMAX];
-		static_for<NMAX>([&](auto I) { v[I] = buf_load(rsrc, voff, min((int)I, n - 1) * stride4); });
-		__builtin_amdgcn_sched_barrier(0);
+		auto ld = [&](int i) { return buf_load(rsrc, voff, min(i, n - 1) * stride4); };
 
 		// Opaque copies of the line geometry for the store phase: otherwise the
 		// per-step store offsets are recognised as the load offsets and all of
@@ -294,15 +340,16 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 					lf = AXIS == 1 ? faces_of_line(d, 1, x, 0, outer, s) : faces_of_line(d, 2, x, outer, 0, s);
 				}
 			}
-			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko, [&](int i, float val) {
-				const int k = min(i, sn);
-				buf_store(rsrc, voff, k * sstride4, face_value(lf, k, sn + 1, val));
-			});
+			thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+											[&](int i, float val) {
+												const int k = min(i, sn);
+												buf_store(rsrc, voff, k * sstride4, face_value(lf, k, sn + 1, val));
+											});
 		}
 		else
 		{
-			thomas_regs<NMAX, ACC>(v, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
-								   [&](int i, float val) { buf_store(rsrc, voff, min(i, sn) * sstride4, val); });
+			thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+											[&](int i, float val) { buf_store(rsrc, voff, min(i, sn) * sstride4, val); });
 		}
 	}
 }

@@ -71,17 +71,26 @@ __device__ __forceinline__ void twist_half(const TwistLine& L, double* lds)
 		// values never reach a real row (the factors that couple them are the
 		// table's zero prefix, and their results are not stored)
 		const int pad = NH - m;
-		static_for<NH>([&](auto J) {
+		auto ld = [&](auto J) {
 			constexpr int j = J;
 			if constexpr (j < kTwistPrefix)
 				v[j] = buf_load(L.rsrc, voff, max(j - pad, 0) * stride4);
 			else
 				v[j] = buf_load(L.rsrc, voff, (j - pad) * stride4);
-		});
-		__builtin_amdgcn_sched_barrier(0);
+		};
+		// loads interleaved with the elimination (as in thomas_regs_streamed):
+		// kLoadAhead slots first, then kTwistChunk more before each group
+		static_for<(NH < kLoadAhead ? NH : kLoadAhead)>([&](auto J) { ld(J); });
 		const double* lb = lds + L.tb - pad; // lb[j] = b'_{j-pad}
-		prev = (double)v[0];
+		prev = 0;
 		static_for<nchunks>([&](auto C) {
+			static_for<kTwistChunk>([&](auto K) {
+				constexpr int j = kLoadAhead + C * kTwistChunk + K;
+				if constexpr (j < NH)
+					ld(std::integral_constant<int, j> {});
+			});
+			if constexpr (C == 0)
+				prev = (double)v[0];
 			static_for<kTwistChunk>([&](auto K) {
 				constexpr int i = 1 + C * kTwistChunk + K;
 				if constexpr (i < NH)
@@ -97,19 +106,27 @@ __device__ __forceinline__ void twist_half(const TwistLine& L, double* lds)
 	{
 		// rows m..n-1 in slots 0..nb-1, eliminated from the last row upwards;
 		// padding slots (>= nb) load row n-1 (decoupled as above)
-		static_for<NH>([&](auto J) {
+		auto ld = [&](auto J) {
 			constexpr int j = J;
 			if constexpr (j >= NH - kTwistPrefix - 1)
 				v[j] = buf_load(L.rsrc, voff, min(m + j, n - 1) * stride4);
 			else
 				v[j] = buf_load(L.rsrc, voff, (m + j) * stride4);
-		});
-		__builtin_amdgcn_sched_barrier(0);
+		};
+		// loads interleaved with the elimination, from the last slot down
+		static_for<(NH < kLoadAhead ? NH : kLoadAhead)>([&](auto J) { ld(std::integral_constant<int, NH - 1 - J> {}); });
 		// slot i couples to slot i+1 through e_{k-1}, k = nb-1-i:
 		// ub[NH-2-i] = b'_{nb-2-i}
 		const double* ub = lds + L.tb - (NH - nb);
-		prev = (double)v[NH - 1];
+		prev = 0;
 		static_for<nchunks>([&](auto C) {
+			static_for<kTwistChunk>([&](auto K) {
+				constexpr int j = NH - 1 - (kLoadAhead + C * kTwistChunk + K);
+				if constexpr (j >= 0)
+					ld(std::integral_constant<int, j> {});
+			});
+			if constexpr (C == 0)
+				prev = (double)v[NH - 1];
 			static_for<kTwistChunk>([&](auto K) {
 				constexpr int i = NH - 2 - (C * kTwistChunk + K);
 				if constexpr (i >= 0)
