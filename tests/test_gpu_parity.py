@@ -161,6 +161,32 @@ def test_stochastic_rounding_distribution():
     assert np.array_equal(d.field().reshape(-1, 4), np.broadcast_to(exact.astype(np.float32), (1000, 4)))
 
 
+@pytest.mark.parametrize("last", ["none", "bulk", "cells", "internalized"])
+def test_step_count_advances(last):
+    """The rounding keys change every step whichever stage ends it: a domain
+    restarted from another's field after one step, with its step count at 0,
+    rounds its next step differently; the uninterrupted run does not."""
+    internalized = last == "internalized"
+    if internalized:
+        p, f0, dirs, bulk, ag = sc.config4(seed=12, n_cells=4000, nx=24, ny=24, nz=12)
+    else:
+        p, f0, dirs, bulk, ag = sc.config2(side=24, S=3, n_cells=400, seed=9, with_cells=last == "cells",
+                                           with_bulk=last == "bulk")
+
+    def make(field):
+        return gpu_domain(p, field=field, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized,
+                          reduce="sorted")
+
+    a, c = make(f0), make(f0)
+    a.step(1)
+    b = make(a.field())
+    a.step(1)
+    b.step(1)
+    c.step(2)
+    assert np.array_equal(a.field(), c.field())
+    assert not np.array_equal(a.field(), b.field())
+
+
 def test_stochastic_rounding_unbiased_3d():
     """Three roundings per step (x scan, y and z register lines): E[field] is
     the exact c0*b^3, so the voxel mean matches it to a few standard errors
