@@ -33,13 +33,20 @@ constexpr int kPipeMinLine = 192;
 // 151 / 203 / 121; 160^3 x 4: 41.2 / - / 40.2; 100^3 x 8: 37.1 / 39.5 / 28.7;
 // 32 x 256^2 x 4: 23.2 / - / 22.4; 8 x 512^2 x 4: 35.5 / - / 22.6; and slower
 // for 64 x 128^2 x 4: 16.3 / - / 17.3, 64 x 128^2 x 8: 25.6 / - / 31.0,
-// 16 x 256^2 x 8: 23.8 / - / 32.1.  Shapes the chosen kernel does not take
-// fall back to the strided lines.
+// 16 x 256^2 x 8: 23.8 / - / 32.1.  With 4-byte accesses (S = 3, 5, 6, 7;
+// strided / chunked, us): 200^3 x 3: 83.7 / 39.9; 128^3 x 3: 31.8 / 25.5;
+// 32 x 256^2 x 3: 27.7 / 23.2; 200^3 x 6: 120 / 76.7; 192^3 x 5: 96.4 / 76.3;
+// 96 x 128^2 x 5: 36.3 / 31.4; and slower for 16 x 256^2 x 6: 21.9 / 25.0,
+// 48 x 160^2 x 5: 23.3 / 24.6, 192^3 x 7: 108 / 128, 64 x 128^2 x 7:
+// 25.3 / 28.4.  Shapes the chosen kernel does not take fall back to the
+// strided lines.
 int default_x_sweep(int nx, int S)
 {
 	if (S <= 2 || nx > bfvm::kRegMaxN)
 		return (S == 2 && nx < 96) ? BFVM_X_TRANSPOSE : BFVM_X_SCAN;
 	if ((S == 4 || S == 8) && bfvm::xt_bucket(nx) && (nx >= 96 || (S == 4 && nx <= 32)))
+		return BFVM_X_CHUNKED;
+	if (bfvm::xg_bucket(nx, S) && (S == 3 || ((S == 5 || S == 6) && nx >= 96)))
 		return BFVM_X_CHUNKED;
 	return BFVM_X_STRIDED;
 }
@@ -389,6 +396,14 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 			t.push_back({ dom, r0, std::min(kScanRowsPerTile, rows - r0), 0 });
 		return;
 	}
+	// chunked 4-byte x kernel: the rows of 4 waves from the first row
+	if (axis == 0 && variant == kSweepXG)
+	{
+		const int rows = ny * nz, per_tile = xg_rows(S) * (kSweepThreads / 64);
+		for (int r0 = 0; r0 < rows; r0 += per_tile)
+			t.push_back({ dom, r0, 0, 0 });
+		return;
+	}
 	// x tiles of whole rows (LDS-staged kernels): first row, row count
 	if (axis == 0 && variant != kSweepReg && variant != kSweepXT)
 	{
@@ -453,6 +468,12 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepXT;
 				c->nmax[axis] = xt_bucket(n);
 			}
+			else if (axis == 0 && xs == BFVM_X_CHUNKED && xg_bucket(n, c->S) && c->fp64 &&
+					 c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
+			{
+				c->variant[axis] = kSweepXG;
+				c->nmax[axis] = xg_bucket(n, c->S);
+			}
 			else if (axis == 0 && xs == BFVM_X_SCAN && vpl && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepScan;
@@ -477,7 +498,8 @@ int build_tiles(bfvm_ctx* c)
 			// register kernels pad the factor rows to their bucket (the twisted
 			// ones also carry a 16-entry zero prefix per substrate)
 			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows ||
-								c->variant[axis] == kSweepPipe || c->variant[axis] == kSweepXT;
+								c->variant[axis] == kSweepPipe || c->variant[axis] == kSweepXT ||
+								c->variant[axis] == kSweepXG;
 			c->fac_cap[axis] = c->variant[axis] == kSweepTwist ? (c->nmax[axis] + 16) * c->S
 															   : (padded ? c->nmax[axis] : n) * c->S;
 			if (c->variant[axis] == kSweepPipe)
@@ -757,6 +779,7 @@ struct bfvm_batch
 		bool fp64 = true, faces = false;
 		int fac_cap = 0, grid = 0, ntiles = 0;
 		size_t lds = 0;
+		int64_t units = 0; // voxel*substrates swept: the launch's share of the axis
 		PipeGeometry pg;
 		std::vector<int> doms;
 		dvec<TileRef> tiles;
@@ -883,7 +906,8 @@ int batch_prepare(bfvm_batch* b)
 				continue;
 			const int v = c->variant[axis];
 			// kernels templated on S: the x scan, the pipelined x kernel
-			const int keyS = (v == kSweepScan || v == kSweepXT || (v == kSweepPipe && axis == 0)) ? c->S : 0;
+			const int keyS =
+				(v == kSweepScan || v == kSweepXT || v == kSweepXG || (v == kSweepPipe && axis == 0)) ? c->S : 0;
 			groups[{ v, c->nmax[axis], c->fp64 ? 1 : 0, keyS }].push_back((int)i);
 		}
 		for (auto& [key, doms] : groups)
@@ -903,6 +927,7 @@ int batch_prepare(bfvm_batch* b)
 				L->faces |= axis == c->dims - 1 && c->face_mask != 0;
 				maxS = std::max(maxS, c->S);
 				lines += axis == 0 ? c->n[1] * c->n[2] * c->S : c->V / c->n[axis] * c->S;
+				L->units += c->V * c->S;
 			}
 			L->S = maxS;
 			std::vector<TileRef> t;
@@ -1014,9 +1039,12 @@ int batch_enqueue_step(bfvm_batch* b)
 				g.push_back(L.get());
 		if (g.empty())
 			continue;
-		// biggest launches first, dealt round-robin over the streams
-		std::stable_sort(g.begin(), g.end(), [](auto* a, auto* c) { return a->ntiles > c->ntiles; });
+		// biggest launches first (by the data they sweep: the sweeps are
+		// memory-bound), each on the stream with the least work so far, so
+		// that the streams finish the axis together
+		std::stable_sort(g.begin(), g.end(), [](auto* a, auto* c) { return a->units > c->units; });
 		const int lanes = std::min<int>(K, (int)g.size());
+		int64_t load[K] = {};
 		if (lanes > 1)
 		{
 			if (hipError_t e = hipEventRecord(b->ev_fork, b->stream); e != hipSuccess)
@@ -1028,7 +1056,8 @@ int batch_enqueue_step(bfvm_batch* b)
 		for (size_t j = 0; j < g.size(); j++)
 		{
 			bfvm_batch::SweepLaunch* L = g[j];
-			const int k = (int)(j % lanes);
+			const int k = (int)(std::min_element(load, load + lanes) - load);
+			load[k] += L->units;
 			hipStream_t st = k == 0 ? b->stream : b->lane[k - 1];
 			hipError_t e;
 			if (L->variant == kSweepPipe)

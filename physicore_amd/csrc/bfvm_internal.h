@@ -205,7 +205,8 @@ enum SweepVariant
 	kSweepRows = 3, // x only: rows staged in LDS, one line per lane in VGPRs
 	kSweepPipe = 4,	// register lines fed by buffer-to-LDS prefetch of the next tile (FP64)
 	kSweepTwist = 5, // y/z lines of 257..512: two lanes per line, twisted elimination (FP64)
-	kSweepXT = 6	 // x, S in {4, 8}: register lines through a chunked LDS transpose (FP64)
+	kSweepXT = 6,	 // x, S in {4, 8}: register lines through a chunked LDS transpose (FP64)
+	kSweepXG = 7	 // x, S in {3, 5, 6, 7}: the same with 4-byte accesses (FP64)
 };
 
 // Geometry of the pipelined sweeps (kSweepPipe): the per-wave tile buffer in
@@ -247,6 +248,17 @@ constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column
 // and the recurrence state exceed 256 VGPRs.
 constexpr int kXtMaxN = 200;
 int xt_bucket(int n);
+// Row bucket of the 4-byte chunked x kernel (S = 3, 5, 6, 7): n rounded up to
+// 16, and 200 for rows of 193..200 when S is 3 or 6; 0 beyond that (the line,
+// its staging, the per-lane offsets and the recurrence state exceed 256 VGPRs;
+// S = 5 and 7 keep more offsets).
+constexpr int kXgMaxN = 200;
+__host__ __device__ constexpr int xg_max_n(int S) { return S == 5 || S == 7 ? 192 : kXgMaxN; }
+int xg_bucket(int n, int S);
+// Rows per wave of that kernel: as many as fit 64 lanes, even for odd S (a
+// 32-voxel chunk of the rows is then a whole number of 64-float loads); a
+// tile holds the rows of kSweepThreads/64 waves.
+__host__ __device__ constexpr int xg_rows(int S) { return S % 2 ? (64 / S) & ~1 : 64 / S; }
 
 // LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);

@@ -4,6 +4,7 @@
 // (separate translation units so they compile in parallel).
 #include "bfvm_pipe_kernels.h"
 #include "bfvm_twist_kernels.h"
+#include "bfvm_xg_kernels.h"
 #include "bfvm_xt_kernels.h"
 
 #include <algorithm>
@@ -24,6 +25,14 @@ kernel_fn pick_xt4a(int nmax);
 kernel_fn pick_xt4b(int nmax);
 kernel_fn pick_xt8a(int nmax);
 kernel_fn pick_xt8b(int nmax);
+kernel_fn pick_xg3a(int nmax);
+kernel_fn pick_xg3b(int nmax);
+kernel_fn pick_xg5a(int nmax);
+kernel_fn pick_xg5b(int nmax);
+kernel_fn pick_xg6a(int nmax);
+kernel_fn pick_xg6b(int nmax);
+kernel_fn pick_xg7a(int nmax);
+kernel_fn pick_xg7b(int nmax);
 kernel_fn pick_twist_y_a(int nmax, bool faces);
 kernel_fn pick_twist_y_b(int nmax, bool faces);
 kernel_fn pick_twist_y_c(int nmax, bool faces);
@@ -113,6 +122,25 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		const bool lo = nmax <= 96;
 		return S == 4 ? (lo ? pick_xt4a(nmax) : pick_xt4b(nmax)) : (lo ? pick_xt8a(nmax) : pick_xt8b(nmax));
 	}
+	if (variant == kSweepXG)
+	{
+		if (axis != 0 || !fp64)
+			return nullptr;
+		const bool lo = nmax <= 96;
+		switch (S)
+		{
+		case 3:
+			return lo ? pick_xg3a(nmax) : pick_xg3b(nmax);
+		case 5:
+			return lo ? pick_xg5a(nmax) : pick_xg5b(nmax);
+		case 6:
+			return lo ? pick_xg6a(nmax) : pick_xg6b(nmax);
+		case 7:
+			return lo ? pick_xg7a(nmax) : pick_xg7b(nmax);
+		default:
+			return nullptr;
+		}
+	}
 	if (variant == kSweepTwist)
 	{
 		if (axis == 0 || !fp64)
@@ -156,6 +184,13 @@ int xt_bucket(int n)
 	return b % 32 == 24 ? b + 8 : b;
 }
 
+int xg_bucket(int n, int S)
+{
+	if ((S != 3 && S != 5 && S != 6 && S != 7) || n < 1 || n > xg_max_n(S))
+		return 0;
+	return n > 192 ? kXgMaxN : ((n + 15) / 16) * 16;
+}
+
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 {
 	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
@@ -169,6 +204,8 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 		return fac + (size_t)sweep::kTwistExchange * sizeof(double);
 	if (variant == kSweepXT)
 		return fac + (size_t)(kSweepThreads / 64) * sweep::xt_buffer_floats(S) * sizeof(float);
+	if (variant == kSweepXG)
+		return fac + (size_t)(kSweepThreads / 64) * sweep::xg_buffer_floats(S) * sizeof(float);
 	return fac;
 }
 
@@ -194,6 +231,18 @@ hipError_t init_sweep_kernels()
 				if (e != hipSuccess)
 					return e;
 			}
+	for (int S : { 3, 5, 6, 7 })
+		for (int nmax = 16; nmax <= xg_max_n(S); nmax += 8)
+		{
+			if (xg_bucket(nmax, S) != nmax)
+				continue;
+			sweep::kernel_fn k = sweep::pick(0, kSweepXG, nmax, true, false, S);
+			if (!k)
+				return hipErrorInvalidValue;
+			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+			if (e != hipSuccess)
+				return e;
+		}
 	for (int S : { 4, 8 })
 		for (int nmax = 8; nmax <= kXtMaxN; nmax += 8)
 		{

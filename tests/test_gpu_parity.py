@@ -89,14 +89,19 @@ def test_diffusion_step_matches_oracle(dims, grid, S, quirks, variant):
     assert err.max() < 2e-6, err
 
 
-# rows for the chunked x kernel: every tail chunk (8, 16, 32 voxels and a
-# bucket rounded up past a 24-voxel tail), row counts that leave a partial last
-# wave (64/S rows per wave), exactly one wave, 2-D, and shapes it does not take
-# (one row, S = 2) where the strided kernel runs instead
+# rows for the chunked x kernels: every tail chunk (8, 16, 24, 32 voxels; for
+# 16-byte accesses a bucket rounded up past a 24-voxel tail), row counts that
+# leave a partial last wave (64/S rows per wave, an even number for odd S),
+# exactly one wave, 2-D, the longest rows of each S, and shapes they do not
+# take (too few rows, rows past the bucket limit, S = 2) where the strided
+# kernel runs instead
 CHUNK_SHAPES = [
     (3, (200, 13, 3), 4), (3, (184, 9, 2), 8), (3, (161, 5, 4), 4), (3, (8, 17, 3), 8),
     (3, (40, 16, 1), 4), (2, (97, 23, 1), 4), (3, (59, 7, 3), 8), (3, (128, 8, 1), 8),
     (3, (192, 31, 2), 4), (1, (150, 1, 1), 4), (3, (60, 10, 10), 2),
+    (3, (200, 9, 5), 3), (3, (193, 11, 2), 6), (3, (185, 13, 2), 5), (3, (192, 5, 2), 7),
+    (3, (200, 5, 2), 7), (2, (24, 20, 1), 3), (3, (8, 3, 3), 5), (3, (100, 7, 9), 3),
+    (3, (37, 6, 4), 6), (3, (70, 9, 3), 7), (3, (120, 4, 7), 5), (2, (16, 57, 1), 6),
 ]
 
 
