@@ -340,11 +340,20 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 					lf = AXIS == 1 ? faces_of_line(d, 1, x, 0, outer, s) : faces_of_line(d, 2, x, outer, 0, s);
 				}
 			}
+			// face_value's precedence without its per-element chain: a line
+			// lying in a face (lo, hi) takes that value everywhere (one select
+			// per element), and the line's own faces are stored at its ends
+			// afterwards (a lane's later store to a voxel wins; hi outranks them)
+			const bool lohi = lf.lo || lf.hi;
+			const float fv = lf.hi ? lf.hi_v : lf.lo_v;
 			thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
 											[&](int i, float val) {
-												const int k = min(i, sn);
-												buf_store(rsrc, voff, k * sstride4, face_value(lf, k, sn + 1, val));
+												buf_store(rsrc, voff, min(i, sn) * sstride4, lohi ? fv : val);
 											});
+			if (!lf.hi && lf.has_min)
+				buf_store(rsrc, voff, 0, lf.min_v);
+			if (!lf.hi && lf.has_max)
+				buf_store(rsrc, voff, sn * sstride4, lf.max_v);
 		}
 		else
 		{
