@@ -30,6 +30,7 @@ namespace sweep {
 
 constexpr int kXgChunk = 32; // voxels per transposed chunk
 constexpr int kXgGroup = 4;	 // recurrence steps per scheduling group (bounds the factor prefetch)
+constexpr int kXgStoreBatch = 8; // LDS reads in flight ahead of their stores
 
 __host__ __device__ constexpr int xg_buffer_floats(int S) { return xg_rows(S) * 33 * S; } // one wave's buffer
 __host__ __device__ constexpr int xg_gcd(int a, int b) { return b == 0 ? a : xg_gcd(b, a % b); }
@@ -224,36 +225,54 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xg_kernel(const DomainDev
 				__builtin_amdgcn_sched_barrier(0);
 			});
 			wave_lds_fence();
+			// the chunk's LDS reads are issued kXgStoreBatch at a time before
+			// their stores: a read followed by its store would wait out every LDS
+			// round trip in turn
 			if constexpr (k < NCH - 1)
 			{
-				if (own == 0)
-					static_for<NQ>([&](auto Q) {
-						constexpr int q = Q;
-						const float o = buf[loff[q % P] + (q / P) * RP * PITCH];
-						buf_store(rsrc, voff[q % P], ((srow0 + (q / P) * RP) * sL + k * CW) * (int)sizeof(float), o);
-						if constexpr (q % 8 == 7)
-							__builtin_amdgcn_sched_barrier(0);
+				static_for<(NQ + kXgStoreBatch - 1) / kXgStoreBatch>([&](auto B) {
+					constexpr int q0 = B * kXgStoreBatch;
+					constexpr int nb = NQ - q0 < kXgStoreBatch ? NQ - q0 : kXgStoreBatch;
+					float o[nb];
+					static_for<nb>([&](auto J) {
+						constexpr int q = q0 + J;
+						o[J] = buf[loff[q % P] + (q / P) * RP * PITCH];
 					});
-				else
-					// the last wave of the domain: only rows from `own` on
-					static_for<NQ>([&](auto Q) {
-						constexpr int q = Q;
-						const int u = (q % P) * 64 + lane;
-						const float o = buf[loff[q % P] + (q / P) * RP * PITCH];
-						if ((q / P) * RP + u / CW >= own)
+					if (own == 0)
+						static_for<nb>([&](auto J) {
+							constexpr int q = q0 + J;
 							buf_store(rsrc, voff[q % P], ((srow0 + (q / P) * RP) * sL + k * CW) * (int)sizeof(float),
-									  o);
-					});
+									  o[J]);
+						});
+					else
+						// the last wave of the domain: only rows from `own` on
+						static_for<nb>([&](auto J) {
+							constexpr int q = q0 + J;
+							const int u = (q % P) * 64 + lane;
+							if ((q / P) * RP + u / CW >= own)
+								buf_store(rsrc, voff[q % P],
+										  ((srow0 + (q / P) * RP) * sL + k * CW) * (int)sizeof(float), o[J]);
+						});
+					__builtin_amdgcn_sched_barrier(0);
+				});
 			}
 			else
+			{
+				float o[NQL];
+				static_for<NQL>([&](auto Q) {
+					constexpr int q = Q;
+					const Tail tl = tail(q);
+					o[q] = buf[tl.row * PITCH + tl.c];
+				});
+				__builtin_amdgcn_sched_barrier(0);
 				static_for<NQL>([&](auto Q) {
 					constexpr int q = Q;
 					const Tail tl = tail(q);
 					const int col = k * CW + tl.c;
-					const float o = buf[tl.row * PITCH + tl.c];
 					if (tl.valid && col < nS && tl.row >= own)
-						buf_store(rsrc, ((srow0 + tl.row) * sL + col) * (int)sizeof(float), 0, o);
+						buf_store(rsrc, ((srow0 + tl.row) * sL + col) * (int)sizeof(float), 0, o[q]);
 				});
+			}
 			wave_lds_fence();
 			__builtin_amdgcn_sched_barrier(0);
 		});

@@ -197,29 +197,33 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xt_kernel(const DomainDev
 				__builtin_amdgcn_sched_barrier(0);
 			});
 			// lane-derived offsets recomputed here (an opaque lane index) rather
-			// than kept alive since the loads; two read/store pairs per
-			// scheduling group bound the staging registers
+			// than kept alive since the loads.  All of the chunk's LDS reads are
+			// issued before its stores (the line registers of the later chunks
+			// are free by now): a read followed by its store would wait out
+			// every LDS round trip in turn
 			wave_lds_fence();
 			int ln = lane;
 			asm volatile("" : "+v"(ln));
 			const int vo = CV == kXtChunk ? sfull : stail;
+			u32x4 w[CV / 4];
 			static_for<CV / 4>([&](auto Q) {
 				const int r = Q * (64 / W4) + ln / W4;
 				const float4 o = *reinterpret_cast<const float4*>(buf + r * PITCH + (ln % W4) * 4);
-				u32x4 w;
-				w.x = __builtin_bit_cast(unsigned, o.x);
-				w.y = __builtin_bit_cast(unsigned, o.y);
-				w.z = __builtin_bit_cast(unsigned, o.z);
-				w.w = __builtin_bit_cast(unsigned, o.w);
+				w[Q].x = __builtin_bit_cast(unsigned, o.x);
+				w[Q].y = __builtin_bit_cast(unsigned, o.y);
+				w[Q].z = __builtin_bit_cast(unsigned, o.z);
+				w[Q].w = __builtin_bit_cast(unsigned, o.w);
+			});
+			__builtin_amdgcn_sched_barrier(0);
+			static_for<CV / 4>([&](auto Q) {
+				const int r = Q * (64 / W4) + ln / W4;
 				const int so = (Q * (64 / W4) * sL + k * kXtChunk * S) * (int)sizeof(float);
 				// padding voxels (column >= n*S) exist only in the last chunks of the bucket
 				bool keep = r >= own;
 				if constexpr ((k + 1) * kXtChunk > NMAX - 16)
 					keep = keep && k * kXtChunk * S + (ln % W4) * 4 < nS;
 				if (keep)
-					__builtin_amdgcn_raw_buffer_store_b128(w, rsrc, vo, so, 0);
-				if constexpr (Q % 2 == 1)
-					__builtin_amdgcn_sched_barrier(0);
+					__builtin_amdgcn_raw_buffer_store_b128(w[Q], rsrc, vo, so, 0);
 			});
 			wave_lds_fence();
 			__builtin_amdgcn_sched_barrier(0);
