@@ -124,21 +124,39 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int voff, in
 // and SGPRs; latency is hidden across the resident waves instead.
 constexpr int kChunk = 16;
 
-// Back substitution of thomas_regs (v holds the rounded forward values).
+// Back substitution of thomas_regs (v holds the rounded forward values).  The
+// factors are read from LDS one group of kBackGroup steps ahead: the backward
+// steps have no memory loads to hide an LDS round trip behind.
+constexpr int kBackGroup = 4;
+
 template <int NMAX, typename ACC, typename Out>
 __device__ __forceinline__ void thomas_back(float (&v)[NMAX], const double* __restrict__ fb, double c, LineKey ko,
 											Out out)
 {
-	constexpr int nchunks = (NMAX - 1 + kChunk - 1) / kChunk;
+	constexpr int ngroups = (NMAX - 1 + kBackGroup - 1) / kBackGroup;
 	const ACC cc = (ACC)c;
 	ACC nxt = (ACC)v[NMAX - 1] * (ACC)fb[NMAX - 1];
 	out(NMAX - 1, field_round<ACC>(nxt, ko, weyl29(NMAX - 1)));
-	static_for<nchunks>([&](auto C) {
-		static_for<kChunk>([&](auto K) {
-			constexpr int i = NMAX - 2 - (C * kChunk + K);
+	// step g*kBackGroup + K solves element NMAX - 2 - (g*kBackGroup + K)
+	auto fetch = [&](auto G, double (&f)[kBackGroup]) {
+		static_for<kBackGroup>([&](auto K) {
+			constexpr int i = NMAX - 2 - (G * kBackGroup + K);
+			if constexpr (i >= 0)
+				f[K] = fb[i];
+		});
+	};
+	double fnext[kBackGroup];
+	fetch(std::integral_constant<int, 0> {}, fnext);
+	static_for<ngroups>([&](auto G) {
+		double fcur[kBackGroup];
+		static_for<kBackGroup>([&](auto K) { fcur[K] = fnext[K]; });
+		if constexpr (G + 1 < ngroups)
+			fetch(std::integral_constant<int, G + 1> {}, fnext);
+		static_for<kBackGroup>([&](auto K) {
+			constexpr int i = NMAX - 2 - (G * kBackGroup + K);
 			if constexpr (i >= 0)
 			{
-				nxt = ((ACC)v[i] + cc * nxt) * (ACC)fb[i];
+				nxt = ((ACC)v[i] + cc * nxt) * (ACC)fcur[K];
 				out(i, field_round<ACC>(nxt, ko, weyl29(i)));
 			}
 		});
