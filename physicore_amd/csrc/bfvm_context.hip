@@ -191,6 +191,13 @@ struct bfvm_ctx
 	dvec<double> bulk_a, bulk_b;
 	dvec<TileRef> bulk_tiles; // sparse tiles of the bulk list
 	int n_bulk_tiles = 0;
+	// voxels in both the bulk and the cell apply lists: voxel -> bulk entry
+	// ([V]), apply entry per bulk entry, bulk flag per apply entry; `overlap`
+	// is true while they match both lists (bulk and cells then run as one
+	// launch, stages_kernel)
+	dvec<int32_t> bulk_map, bulk_cell;
+	dvec<uint8_t> cell_in_bulk;
+	bool overlap = false;
 
 	int64_t n_agents = 0;
 	dvec<double> positions, secretion, saturation, uptake, net_export, internalized_d, fraction_released, volumes;
@@ -329,6 +336,8 @@ int sync_domain(bfvm_ctx* c)
 	d.bulk_vox = c->bulk_vox.p;
 	d.bulk_a = c->bulk_a.p;
 	d.bulk_b = c->bulk_b.p;
+	d.bulk_cell = c->bulk_cell.p;
+	d.cell_in_bulk = c->cell_in_bulk.p;
 	d.n_agents = c->n_agents;
 	d.positions = c->positions.p;
 	d.secretion = c->secretion.p;
@@ -566,17 +575,37 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
 	if (c->dir_count)
 		TRY_HIP(c, launch_dirichlet_interior(c->ddom.p, c->dir_refs.p, c->dir_count, c->stream));
 	TRY_HIP(c, mark(4));
-	if (c->bulk_count)
+	// bulk and cell apply as one launch when both run (voxels in both lists
+	// are applied in order by the bulk side); the stage timer then counts the
+	// cell apply with bulk
+	const bool both = c->bulk_count && c->n_bulk_tiles > 0 && c->n_owners > 0 && c->n_cell_tiles > 0 && c->overlap;
+	if (both)
+		TRY_HIP(c, launch_stages(c->ddom.p, c->bulk_tiles.p, c->n_bulk_tiles, c->cell_tiles.p, c->n_cell_tiles,
+								 c->stream));
+	else if (c->bulk_count)
 		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_tiles.p, c->n_bulk_tiles, c->stream));
 	TRY_HIP(c, mark(5));
 	if (c->n_owners > 0)
 	{
-		TRY_HIP(c, launch_cells_apply(c->ddom.p, c->cell_tiles.p, c->n_cell_tiles, c->stream));
+		if (!both)
+			TRY_HIP(c, launch_cells_apply(c->ddom.p, c->cell_tiles.p, c->n_cell_tiles, c->stream));
 		if (c->internalized && c->conflict)
 			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->int_tiles.p, c->n_int_tiles, c->stream));
 	}
 	TRY_HIP(c, launch_step_tick(c->ddom.p, 1, c->stream));
 	TRY_HIP(c, mark(6));
+	return BFVM_OK;
+}
+
+// The bulk/apply overlap maps for the current lists (after a recompute or a
+// new bulk list); without both lists the stages run as separate launches.
+int rebuild_overlap(bfvm_ctx* c)
+{
+	c->overlap = false;
+	if (c->bulk_count == 0 || c->n_owners == 0 || c->recompute_pending || !c->bulk_map.p)
+		return BFVM_OK;
+	TRY_HIP(c, launch_bulk_overlap(c->ddom.p, c->bulk_map.p, c->bulk_cell.p, c->bulk_count, c->n_owners, c->stream));
+	c->overlap = true;
 	return BFVM_OK;
 }
 
@@ -595,6 +624,7 @@ int recompute(bfvm_ctx* c)
 	c->recompute_pending = false;
 	c->n_owners = 0;
 	c->conflict = false;
+	c->overlap = false;
 	const int64_t n = c->n_agents;
 	if (n == 0)
 		return BFVM_OK;
@@ -649,6 +679,7 @@ int recompute(bfvm_ctx* c)
 		auto t = sparse_tiles(0, c->n_owners);
 		c->n_cell_tiles = (int)t.size();
 		TRY_HIP(c, c->cell_tiles.upload(t.data(), t.size(), s)); // synchronizes s
+		return rebuild_overlap(c);
 	}
 	return BFVM_OK;
 }
@@ -1403,7 +1434,14 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 	auto t = sparse_tiles(0, count);
 	c->n_bulk_tiles = (int)t.size();
 	TRY_HIP(c, c->bulk_tiles.upload(t.data(), t.size(), c->stream));
-	return sync_domain(c);
+	if (count > 0)
+	{
+		TRY_HIP(c, c->bulk_map.alloc((size_t)c->V));
+		TRY_HIP(c, c->bulk_cell.alloc((size_t)count));
+		TRY_HIP(c, launch_bulk_map(c->bulk_vox.p, count, c->bulk_map.p, c->V, c->stream));
+	}
+	int rc = sync_domain(c);
+	return rc ? rc : rebuild_overlap(c);
 }
 
 int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const double* secretion,
@@ -1451,12 +1489,14 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->pden.alloc(nS));
 	TRY_HIP(c, c->pfac.alloc(nS));
 	TRY_HIP(c, c->pvox.alloc(n));
+	TRY_HIP(c, c->cell_in_bulk.alloc(n));
 	TRY_HIP(c, c->slot_of.alloc(n));
 	auto t = sparse_tiles(0, n);
 	c->n_int_tiles = (int)t.size();
 	TRY_HIP(c, c->int_tiles.upload(t.data(), t.size(), c->stream));
 	c->recompute_pending = true;
 	c->n_owners = 0;
+	c->overlap = false;
 	return sync_domain(c);
 }
 

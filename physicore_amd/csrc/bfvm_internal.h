@@ -42,6 +42,11 @@ struct DomainDev
 	const int64_t* bulk_vox;
 	const double* bulk_a;
 	const double* bulk_b;
+	// bulk and cell lists sharing voxels (rebuilt with either list): the apply
+	// list entry at a bulk voxel or -1, and whether an apply entry's voxel is
+	// in the bulk list (stages_kernel applies such voxels from the bulk side)
+	int32_t* bulk_cell; // [bulk count]
+	uint8_t* cell_in_bulk; // [n_owners]
 
 	// agents (FP64, as the reference)
 	int64_t n_agents;
@@ -309,6 +314,16 @@ hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, 
 hipError_t launch_step_tick(const DomainDev* doms, int ndoms, hipStream_t stream);
 hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
+// Bulk and cell apply of one domain in one launch (bulk tiles first), once its
+// overlap maps are current (launch_bulk_overlap).
+hipError_t launch_stages(const DomainDev* dom, const TileRef* bulk_tiles, int n_bulk_tiles, const TileRef* cell_tiles,
+						 int n_cell_tiles, hipStream_t stream);
+// Voxel -> bulk entry map ([V], -1 elsewhere) of the bulk list.
+hipError_t launch_bulk_map(const int64_t* bulk_vox, int64_t count, int32_t* map, int64_t V, hipStream_t stream);
+// The domain's overlap maps (bulk_cell: the domain's, on the device) from the
+// voxel map and its apply list.
+hipError_t launch_bulk_overlap(const DomainDev* dom, const int32_t* map, int32_t* bulk_cell, int64_t bulk_count,
+							   int64_t n_owners, hipStream_t stream);
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
 hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream);
 hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);

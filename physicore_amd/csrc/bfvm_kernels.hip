@@ -298,6 +298,22 @@ __global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_ow
 // per-agent update.  One thread per (occupied voxel, substrate).  Agent
 // activity is fixed between recomputes (new agent arrays always trigger one),
 // so every listed owner is active.
+// The cell update of apply entry k, substrate s, from the density d0 at its
+// voxel; returns the FP32 value to store there.
+__device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, double d0, uint32_t seed, bool fused,
+											 bool nearest)
+{
+	const int S = d.S;
+	const int64_t q = (int64_t)k * S + s;
+	const int64_t e = (int64_t)d.pvox[k] * S + s;
+	const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
+	if (fused)
+		d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
+	else
+		d.dnew[q] = d1;
+	return sr_round(d1, mix32((uint32_t)e ^ seed), nearest);
+}
+
 __global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
 	const TileRef tr = tiles[blockIdx.x];
@@ -309,17 +325,80 @@ __global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const Til
 	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 	{
 		const int k = tr.a + t / S, s = t % S;
-		const int64_t q = (int64_t)k * S + s;
-		const int64_t e = (int64_t)d.pvox[k] * S + s;
-		float* D = d.field + e;
-		const double d0 = (double)*D;
-		const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
-		*D = sr_round(d1, mix32((uint32_t)e ^ seed), nearest);
-		if (fused)
-			d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
-		else
-			d.dnew[q] = d1;
+		float* D = d.field + (int64_t)d.pvox[k] * S + s;
+		*D = cell_update(d, k, s, (double)*D, seed, fused, nearest);
 	}
+}
+
+// Bulk then cell apply of one domain in one launch: blocks below nb take bulk
+// tiles, the rest apply tiles.  A voxel in both lists is done by its bulk
+// thread: the bulk update rounded to FP32, then the cell update of that value,
+// the two stages' arithmetic in the reference's order; its apply thread skips
+// it.  Results equal bulk_kernel followed by cells_apply_kernel.
+__global__ void stages_kernel(const DomainDev* __restrict__ dom, const TileRef* __restrict__ btiles, int nb,
+							  const TileRef* __restrict__ ctiles)
+{
+	const DomainDev& d = *dom;
+	const int S = d.S;
+	const bool nearest = d.round_nearest != 0;
+	const uint32_t cseed = stage_seed(d, kRoundCells);
+	const bool fused = d.compute_internalized && !d.flags[0];
+	if ((int)blockIdx.x < nb)
+	{
+		const TileRef tr = btiles[blockIdx.x];
+		const uint32_t bseed = stage_seed(d, kRoundBulk);
+		for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+		{
+			const int k = tr.a + t / S, s = t % S;
+			const int64_t q = (int64_t)k * S + s;
+			const int64_t e = d.bulk_vox[k] * S + s;
+			float* D = d.field + e;
+			float v = sr_round(((double)*D + d.bulk_a[q]) / d.bulk_b[q], mix32((uint32_t)e ^ bseed), nearest);
+			const int j = d.bulk_cell[k];
+			if (j >= 0)
+				v = cell_update(d, j, s, (double)v, cseed, fused, nearest);
+			*D = v;
+		}
+	}
+	else
+	{
+		const TileRef tr = ctiles[blockIdx.x - nb];
+		for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+		{
+			const int k = tr.a + t / S, s = t % S;
+			if (d.cell_in_bulk[k])
+				continue;
+			float* D = d.field + (int64_t)d.pvox[k] * S + s;
+			*D = cell_update(d, k, s, (double)*D, cseed, fused, nearest);
+		}
+	}
+}
+
+__global__ void bulk_map_kernel(const int64_t* __restrict__ vox, int64_t count, int32_t* map)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < count)
+		map[vox[k]] = (int32_t)k;
+}
+
+__global__ void bulk_cell_clear_kernel(int32_t* bulk_cell, int64_t count)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < count)
+		bulk_cell[k] = -1;
+}
+
+__global__ void bulk_overlap_kernel(const DomainDev* __restrict__ dom, const int32_t* __restrict__ map,
+									int64_t n_owners)
+{
+	const DomainDev& d = *dom;
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n_owners)
+		return;
+	const int32_t b = map[d.pvox[k]];
+	d.cell_in_bulk[k] = b >= 0;
+	if (b >= 0)
+		d.bulk_cell[b] = (int32_t)k;
 }
 
 // compute_internalized (cell_solver.cpp:131-144, :229-243):
@@ -637,6 +716,32 @@ hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int n
 		return hipSuccess;
 	hipLaunchKernelGGL(cells_apply_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
 	return hipGetLastError();
+}
+
+hipError_t launch_stages(const DomainDev* dom, const TileRef* bulk_tiles, int n_bulk_tiles, const TileRef* cell_tiles,
+						 int n_cell_tiles, hipStream_t stream)
+{
+	if (n_bulk_tiles + n_cell_tiles <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(stages_kernel, dim3(n_bulk_tiles + n_cell_tiles), dim3(256), 0, stream, dom, bulk_tiles,
+					   n_bulk_tiles, cell_tiles);
+	return hipGetLastError();
+}
+
+hipError_t launch_bulk_map(const int64_t* bulk_vox, int64_t count, int32_t* map, int64_t V, hipStream_t stream)
+{
+	if (hipError_t e = hipMemsetAsync(map, 0xff, (size_t)V * sizeof(int32_t), stream); e != hipSuccess)
+		return e;
+	return launch1d(bulk_map_kernel, count, 256, stream, bulk_vox, count, map);
+}
+
+hipError_t launch_bulk_overlap(const DomainDev* dom, const int32_t* map, int32_t* bulk_cell, int64_t bulk_count,
+							   int64_t n_owners, hipStream_t stream)
+{
+	if (hipError_t e = launch1d(bulk_cell_clear_kernel, bulk_count, 256, stream, bulk_cell, bulk_count);
+		e != hipSuccess)
+		return e;
+	return launch1d(bulk_overlap_kernel, n_owners, 256, stream, dom, map, n_owners);
 }
 
 hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream)

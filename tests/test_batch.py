@@ -62,6 +62,28 @@ def test_batch_equals_alone_chunked_x():
         assert np.array_equal(a.field(), m.field()), k
 
 
+@pytest.mark.parametrize("internalized", [False, True])
+def test_fused_bulk_and_cells_equal_separate(internalized):
+    """A domain stepped alone runs bulk and cell apply as one launch (voxels in
+    both lists applied by the bulk side); a batch runs them as two.  Fields and
+    internalized amounts are identical, with many cells inside the bulk region."""
+    require_gpu()
+    p, f0, dirs, bulk, ag = sc.config2(side=64, S=3, n_cells=3000, seed=17)
+    supply = np.asarray(bulk[0]).reshape(-1, p.S)
+    vox = np.floor(ag.positions / 20).astype(int)
+    lin = (vox[:, 2] * p.ny + vox[:, 1]) * p.nx + vox[:, 0]
+    inside = (supply[lin] != 0).any(axis=1).sum()
+    assert 100 < inside < len(lin) - 100  # cells inside and outside the bulk region
+    alone = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized, reduce="sorted")
+    mixed = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, internalized=internalized, reduce="sorted")
+    b = bfvm.Batch([mixed])
+    alone.step(6)
+    b.step(6)
+    assert np.array_equal(alone.field(), mixed.field())
+    if internalized:
+        assert np.array_equal(alone.internalized(), mixed.internalized())
+
+
 def test_batch_graph_equals_eager():
     require_gpu()
     scns = _make(2, 9, extras=True)
