@@ -219,10 +219,10 @@ typedef enum bfvm_x_sweep
 						   * (FP64, S in {1, 2, 4, 8}, nx*S % 4 == 0) */
 	BFVM_X_CHUNKED = 5	  /* one line per lane in VGPRs, filled and emptied through an
 						   * LDS transpose of 32-voxel chunks (FP64): 16-byte accesses
-						   * for S in {4, 8}, 4-byte accesses for S in {3, 5, 6, 7};
-						   * rows of up to 200 voxels (192 for S = 5, 7) and at least
-						   * one wave of rows (64/S, even for odd S); other shapes use
-						   * the strided lines */
+						   * for S in {4, 8}, 4-byte accesses otherwise (S <= 7); rows
+						   * of up to 200 voxels (176 for S = 1, 192 for S = 5, 7) and
+						   * at least one wave of rows (64/S, even for odd S); other
+						   * shapes use the strided lines */
 } bfvm_x_sweep;
 
 /* Kernel selection and rounding.  persistent_wgs_per_cu: 0 = the default per

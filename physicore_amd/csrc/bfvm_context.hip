@@ -38,10 +38,15 @@ constexpr int kPipeMinLine = 192;
 // 32 x 256^2 x 3: 27.7 / 23.2; 200^3 x 6: 120 / 76.7; 192^3 x 5: 96.4 / 76.3;
 // 96 x 128^2 x 5: 36.3 / 31.4; and slower for 16 x 256^2 x 6: 21.9 / 25.0,
 // 48 x 160^2 x 5: 23.3 / 24.6, 192^3 x 7: 108 / 128, 64 x 128^2 x 7:
-// 25.3 / 28.4.  Shapes the chosen kernel does not take fall back to the
-// strided lines.
+// 25.3 / 28.4.  For S = 1, 2 (scan or transpose / chunked, us): 200^3 x 2:
+// 48.9 / 37.0; 128^3 x 2: 26.6 / 26.3; 64 x 128^2 x 2: 19.0 / 20.8; S = 1
+// (64 rows per wave: too few waves below ~10^5 rows): 176 x 200^2: 36.4 /
+// 28.7; 160^3: 27.4 / 29.8; 128^3: 18.9 / 28.3.  Shapes the chosen kernel does
+// not take fall back to the strided lines.
 int default_x_sweep(int nx, int S)
 {
+	if (S == 2 && nx >= 160 && bfvm::xg_bucket(nx, S))
+		return BFVM_X_CHUNKED;
 	if (S <= 2 || nx > bfvm::kRegMaxN)
 		return (S == 2 && nx < 96) ? BFVM_X_TRANSPOSE : BFVM_X_SCAN;
 	if ((S == 4 || S == 8) && bfvm::xt_bucket(nx) && (nx >= 96 || (S == 4 && nx <= 32)))

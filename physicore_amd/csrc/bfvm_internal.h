@@ -253,12 +253,12 @@ constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column
 // and the recurrence state exceed 256 VGPRs.
 constexpr int kXtMaxN = 200;
 int xt_bucket(int n);
-// Row bucket of the 4-byte chunked x kernel (S = 3, 5, 6, 7): n rounded up to
-// 16, and 200 for rows of 193..200 when S is 3 or 6; 0 beyond that (the line,
-// its staging, the per-lane offsets and the recurrence state exceed 256 VGPRs;
-// S = 5 and 7 keep more offsets).
+// Row bucket of the 4-byte chunked x kernel (S = 1, 2, 3, 5, 6, 7): n rounded up
+// to 16, and 200 for rows of 193..200 unless S is 1, 5 or 7; 0 beyond
+// xg_max_n(S) (the line, its staging, the per-lane offsets and the recurrence
+// state exceed 256 VGPRs).
 constexpr int kXgMaxN = 200;
-__host__ __device__ constexpr int xg_max_n(int S) { return S == 5 || S == 7 ? 192 : kXgMaxN; }
+__host__ __device__ constexpr int xg_max_n(int S) { return S == 1 ? 176 : (S == 5 || S == 7 ? 192 : kXgMaxN); }
 int xg_bucket(int n, int S);
 // Rows per wave of that kernel: as many as fit 64 lanes, even for odd S (a
 // 32-voxel chunk of the rows is then a whole number of 64-float loads); a

@@ -25,6 +25,10 @@ kernel_fn pick_xt4a(int nmax);
 kernel_fn pick_xt4b(int nmax);
 kernel_fn pick_xt8a(int nmax);
 kernel_fn pick_xt8b(int nmax);
+kernel_fn pick_xg1a(int nmax);
+kernel_fn pick_xg1b(int nmax);
+kernel_fn pick_xg2a(int nmax);
+kernel_fn pick_xg2b(int nmax);
 kernel_fn pick_xg3a(int nmax);
 kernel_fn pick_xg3b(int nmax);
 kernel_fn pick_xg5a(int nmax);
@@ -129,6 +133,10 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		const bool lo = nmax <= 96;
 		switch (S)
 		{
+		case 1:
+			return lo ? pick_xg1a(nmax) : pick_xg1b(nmax);
+		case 2:
+			return lo ? pick_xg2a(nmax) : pick_xg2b(nmax);
 		case 3:
 			return lo ? pick_xg3a(nmax) : pick_xg3b(nmax);
 		case 5:
@@ -186,7 +194,7 @@ int xt_bucket(int n)
 
 int xg_bucket(int n, int S)
 {
-	if ((S != 3 && S != 5 && S != 6 && S != 7) || n < 1 || n > xg_max_n(S))
+	if (S < 1 || S > 7 || S == 4 || n < 1 || n > xg_max_n(S))
 		return 0;
 	return n > 192 ? kXgMaxN : ((n + 15) / 16) * 16;
 }
@@ -231,7 +239,7 @@ hipError_t init_sweep_kernels()
 				if (e != hipSuccess)
 					return e;
 			}
-	for (int S : { 3, 5, 6, 7 })
+	for (int S : { 1, 2, 3, 5, 6, 7 })
 		for (int nmax = 16; nmax <= xg_max_n(S); nmax += 8)
 		{
 			if (xg_bucket(nmax, S) != nmax)

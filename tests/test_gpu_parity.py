@@ -102,6 +102,8 @@ CHUNK_SHAPES = [
     (3, (200, 9, 5), 3), (3, (193, 11, 2), 6), (3, (185, 13, 2), 5), (3, (192, 5, 2), 7),
     (3, (200, 5, 2), 7), (2, (24, 20, 1), 3), (3, (8, 3, 3), 5), (3, (100, 7, 9), 3),
     (3, (37, 6, 4), 6), (3, (70, 9, 3), 7), (3, (120, 4, 7), 5), (2, (16, 57, 1), 6),
+    (3, (176, 70, 1), 1), (3, (177, 70, 1), 1), (3, (33, 65, 2), 1), (2, (100, 64, 1), 1),
+    (3, (200, 33, 2), 2), (3, (17, 40, 1), 2), (2, (129, 31, 1), 2),
 ]
 
 
