@@ -678,6 +678,23 @@ int recompute(bfvm_ctx* c)
 	}
 	c->conflict = f[0] != 0;
 	c->n_owners = f[2];
+	if (c->reduce_mode == BFVM_REDUCE_ATOMIC && c->n_owners > 1)
+	{
+		// the atomics collected the owners in no particular order: sort the
+		// list by voxel so that the per-step apply walks the field in order
+		// (the list order does not change any result)
+		const int64_t m = c->n_owners;
+		TRY_HIP(c, c->keys.alloc(n));
+		TRY_HIP(c, c->vals.alloc(n));
+		TRY_HIP(c, c->tkeys.alloc(n));
+		TRY_HIP(c, c->tvals.alloc(n));
+		TRY_HIP(c, c->hist.alloc(radix_hist_entries(n)));
+		TRY_HIP(c, c->scan_tmp.alloc(std::max(scan_tmp_entries(n), scan_tmp_entries((int64_t)radix_hist_entries(n)))));
+		TRY_HIP(c, launch_cells_owner_keys(c->ddom.p, c->keys.p, c->vals.p, m, s));
+		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, m,
+									bits_for((uint64_t)c->V), s));
+		TRY_HIP(c, hipMemcpyAsync(c->owners.p, c->vals.p, m * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+	}
 	if (c->n_owners > 0)
 	{
 		TRY_HIP(c, launch_cells_pack(c->ddom.p, c->n_owners, s));

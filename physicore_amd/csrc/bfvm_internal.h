@@ -337,6 +337,10 @@ hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stre
 hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream);
 // packs the apply list from owners[0..n_owners) (both modes)
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream);
+// atomic mode: keys/vals = (voxel, owner) of the collected owner list, to sort
+// it into voxel order (radix_sort_pairs)
+hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n_owners,
+								   hipStream_t stream);
 // sorted mode: keys/vals are scratch of n entries (sorted in place by radix_sort_pairs)
 hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
 								  hipStream_t stream);

@@ -273,6 +273,20 @@ __global__ void cells_segment_reduce_kernel(const DomainDev* __restrict__ dp, co
 
 // Packs the apply list in owner-list order: the owner's voxel and its FP64
 // sums, so the per-step kernels read them coalesced.
+// Atomic mode: (voxel, owner agent) of each collected owner, to put the owner
+// list in voxel order (the collection order is the order of the atomics).
+__global__ void cells_owner_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals,
+										int64_t n_owners)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n_owners)
+		return;
+	const DomainDev& d = *dp;
+	const int o = d.owners[k];
+	keys[k] = (uint32_t)d.cell_voxel[o];
+	vals[k] = (uint32_t)o;
+}
+
 __global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_owners)
 {
 	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -750,6 +764,12 @@ hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles
 		return hipSuccess;
 	hipLaunchKernelGGL(cells_internalized_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
 	return hipGetLastError();
+}
+
+hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n_owners,
+								   hipStream_t stream)
+{
+	return launch1d(cells_owner_keys_kernel, n_owners, 256, stream, dom, keys, vals, n_owners);
 }
 
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream)
