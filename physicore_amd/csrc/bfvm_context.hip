@@ -42,11 +42,17 @@ constexpr int kPipeMinLine = 192;
 // 48.9 / 37.0; 128^3 x 2: 26.6 / 26.3; 64 x 128^2 x 2: 19.0 / 20.8; S = 1
 // (64 rows per wave: too few waves below ~10^5 rows): 176 x 200^2: 36.4 /
 // 28.7; 160^3: 27.4 / 29.8; 128^3: 18.9 / 28.3.  Shapes the chosen kernel does
-// not take fall back to the strided lines.
+// not take fall back to the strided lines.  Rows longer than the one-lane
+// chunked kernels hold (strided / scan / two-lane chunked, us): 256 x 160^2 x 3:
+// 108 / 43.4 / 54.6; 240 x 160^2 x 5: 84.7 / 65.9 / 84.8; 256 x 128^2 x 6: 82.0
+// / 54.8 / 76.6; 400 x 400 x 100 x 2: 407 / 74.1 / 74.1; and where the scan does
+// not fit, 300 x 128^2 x 7: 507 (LDS tiles) / - / 101.
 int default_x_sweep(int nx, int S)
 {
 	if (S == 2 && nx >= 160 && bfvm::xg_bucket(nx, S))
 		return BFVM_X_CHUNKED;
+	if (nx > 200 || (S != 4 && S != 8 && nx > bfvm::xg_max_n(S)))
+		return bfvm::xscan_vpl(nx, S) ? BFVM_X_SCAN : BFVM_X_CHUNKED;
 	if (S <= 2 || nx > bfvm::kRegMaxN)
 		return (S == 2 && nx < 96) ? BFVM_X_TRANSPOSE : BFVM_X_SCAN;
 	if ((S == 4 || S == 8) && bfvm::xt_bucket(nx) && (nx >= 96 || (S == 4 && nx <= 32)))
@@ -410,10 +416,12 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 			t.push_back({ dom, r0, std::min(kScanRowsPerTile, rows - r0), 0 });
 		return;
 	}
-	// chunked 4-byte x kernel: the rows of 4 waves from the first row
-	if (axis == 0 && variant == kSweepXG)
+	// chunked 4-byte x kernels: the rows of 4 waves (of 2 wave pairs for the
+	// two-lane kernel) from the first row
+	if (axis == 0 && (variant == kSweepXG || variant == kSweepXGT))
 	{
-		const int rows = ny * nz, per_tile = xg_rows(S) * (kSweepThreads / 64);
+		const int rows = ny * nz;
+		const int per_tile = xg_rows(S) * (kSweepThreads / 64) / (variant == kSweepXGT ? 2 : 1);
 		for (int r0 = 0; r0 < rows; r0 += per_tile)
 			t.push_back({ dom, r0, 0, 0 });
 		return;
@@ -488,6 +496,14 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepXG;
 				c->nmax[axis] = xg_bucket(n, c->S);
 			}
+			// rows too long for one lane: two lanes per line (the top and bottom
+			// halves in two waves)
+			else if (axis == 0 && xs == BFVM_X_CHUNKED && xgt_bucket(n, c->S) && c->fp64 &&
+					 c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
+			{
+				c->variant[axis] = kSweepXGT;
+				c->nmax[axis] = xgt_bucket(n, c->S);
+			}
 			else if (axis == 0 && xs == BFVM_X_SCAN && vpl && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepScan;
@@ -514,7 +530,8 @@ int build_tiles(bfvm_ctx* c)
 			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows ||
 								c->variant[axis] == kSweepPipe || c->variant[axis] == kSweepXT ||
 								c->variant[axis] == kSweepXG;
-			c->fac_cap[axis] = c->variant[axis] == kSweepTwist ? (c->nmax[axis] + 16) * c->S
+			c->fac_cap[axis] = c->variant[axis] == kSweepTwist  ? (c->nmax[axis] + 16) * c->S
+							   : c->variant[axis] == kSweepXGT ? (c->nmax[axis] + kXgtPrefix) * c->S
 															   : (padded ? c->nmax[axis] : n) * c->S;
 			if (c->variant[axis] == kSweepPipe)
 			{
@@ -960,7 +977,9 @@ int batch_prepare(bfvm_batch* b)
 			const int v = c->variant[axis];
 			// kernels templated on S: the x scan, the pipelined x kernel
 			const int keyS =
-				(v == kSweepScan || v == kSweepXT || v == kSweepXG || (v == kSweepPipe && axis == 0)) ? c->S : 0;
+				(v == kSweepScan || v == kSweepXT || v == kSweepXG || v == kSweepXGT || (v == kSweepPipe && axis == 0))
+					? c->S
+					: 0;
 			groups[{ v, c->nmax[axis], c->fp64 ? 1 : 0, keyS }].push_back((int)i);
 		}
 		for (auto& [key, doms] : groups)

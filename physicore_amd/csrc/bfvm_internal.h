@@ -211,7 +211,8 @@ enum SweepVariant
 	kSweepPipe = 4,	// register lines fed by buffer-to-LDS prefetch of the next tile (FP64)
 	kSweepTwist = 5, // y/z lines of 257..512: two lanes per line, twisted elimination (FP64)
 	kSweepXT = 6,	 // x, S in {4, 8}: register lines through a chunked LDS transpose (FP64)
-	kSweepXG = 7	 // x, S in {3, 5, 6, 7}: the same with 4-byte accesses (FP64)
+	kSweepXG = 7,	 // x, S <= 7 but 4: the same with 4-byte accesses (FP64)
+	kSweepXGT = 8	 // x rows past xg_max_n: two lanes per line, twisted, 4-byte chunks (FP64)
 };
 
 // Geometry of the pipelined sweeps (kSweepPipe): the per-wave tile buffer in
@@ -264,6 +265,12 @@ int xg_bucket(int n, int S);
 // 32-voxel chunk of the rows is then a whole number of 64-float loads); a
 // tile holds the rows of kSweepThreads/64 waves.
 __host__ __device__ constexpr int xg_rows(int S) { return S % 2 ? (64 / S) & ~1 : 64 / S; }
+// Half-row bucket of the two-lane chunked x kernel (rows of xg_max_n(S)+1 ..
+// 2*xg_max_n(S) voxels): the top half holds voxels 0..NH-1, NH = ceil(n/2)
+// rounded up to 16 (200 above 192), the bottom half the remaining n - NH >=
+// NH - 31; 0 outside that range.  The factor table carries kXgtPrefix zeros.
+int xgt_bucket(int n, int S);
+constexpr int kXgtPrefix = 32;
 
 // LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);

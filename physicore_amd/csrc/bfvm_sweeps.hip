@@ -5,6 +5,7 @@
 #include "bfvm_pipe_kernels.h"
 #include "bfvm_twist_kernels.h"
 #include "bfvm_xg_kernels.h"
+#include "bfvm_xgt_kernels.h"
 #include "bfvm_xt_kernels.h"
 
 #include <algorithm>
@@ -25,6 +26,12 @@ kernel_fn pick_xt4a(int nmax);
 kernel_fn pick_xt4b(int nmax);
 kernel_fn pick_xt8a(int nmax);
 kernel_fn pick_xt8b(int nmax);
+kernel_fn pick_xgt1(int nmax);
+kernel_fn pick_xgt2(int nmax);
+kernel_fn pick_xgt3(int nmax);
+kernel_fn pick_xgt5(int nmax);
+kernel_fn pick_xgt6(int nmax);
+kernel_fn pick_xgt7(int nmax);
 kernel_fn pick_xg1a(int nmax);
 kernel_fn pick_xg1b(int nmax);
 kernel_fn pick_xg2a(int nmax);
@@ -126,6 +133,28 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		const bool lo = nmax <= 96;
 		return S == 4 ? (lo ? pick_xt4a(nmax) : pick_xt4b(nmax)) : (lo ? pick_xt8a(nmax) : pick_xt8b(nmax));
 	}
+	if (variant == kSweepXGT)
+	{
+		if (axis != 0 || !fp64)
+			return nullptr;
+		switch (S)
+		{
+		case 1:
+			return pick_xgt1(nmax);
+		case 2:
+			return pick_xgt2(nmax);
+		case 3:
+			return pick_xgt3(nmax);
+		case 5:
+			return pick_xgt5(nmax);
+		case 6:
+			return pick_xgt6(nmax);
+		case 7:
+			return pick_xgt7(nmax);
+		default:
+			return nullptr;
+		}
+	}
 	if (variant == kSweepXG)
 	{
 		if (axis != 0 || !fp64)
@@ -199,6 +228,15 @@ int xg_bucket(int n, int S)
 	return n > 192 ? kXgMaxN : ((n + 15) / 16) * 16;
 }
 
+int xgt_bucket(int n, int S)
+{
+	if (S < 1 || S > 7 || S == 4 || n <= xg_max_n(S) || n > 2 * xg_max_n(S))
+		return 0;
+	const int h = (n + 1) / 2;
+	const int nh = h > 192 ? kXgMaxN : ((h + 15) / 16) * 16;
+	return nh <= xg_max_n(S) ? nh : 0;
+}
+
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 {
 	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
@@ -214,6 +252,9 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 		return fac + (size_t)(kSweepThreads / 64) * sweep::xt_buffer_floats(S) * sizeof(float);
 	if (variant == kSweepXG)
 		return fac + (size_t)(kSweepThreads / 64) * sweep::xg_buffer_floats(S) * sizeof(float);
+	if (variant == kSweepXGT)
+		return fac + (size_t)sweep::kXgtExchange * sizeof(double) +
+			   (size_t)(kSweepThreads / 64) * sweep::xg_buffer_floats(S) * sizeof(float);
 	return fac;
 }
 
@@ -239,6 +280,19 @@ hipError_t init_sweep_kernels()
 				if (e != hipSuccess)
 					return e;
 			}
+	for (int S : { 1, 2, 3, 5, 6, 7 })
+		for (int n = xg_max_n(S) + 1; n <= 2 * xg_max_n(S); n++)
+		{
+			const int nh = xgt_bucket(n, S);
+			if (nh == 0 || (n > xg_max_n(S) + 1 && xgt_bucket(n - 1, S) == nh))
+				continue;
+			sweep::kernel_fn k = sweep::pick(0, kSweepXGT, nh, true, false, S);
+			if (!k)
+				return hipErrorInvalidValue;
+			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+			if (e != hipSuccess)
+				return e;
+		}
 	for (int S : { 1, 2, 3, 5, 6, 7 })
 		for (int nmax = 16; nmax <= xg_max_n(S); nmax += 8)
 		{

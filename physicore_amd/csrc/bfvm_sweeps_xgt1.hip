@@ -1,0 +1,28 @@
+// Instantiations of the two-lane chunked x sweep for 1 substrate, rows of
+// 177..352 voxels (half-row buckets).
+#include "bfvm_xgt_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_xgt1(int nmax)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return sweep_xgt_kernel<N, 1>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(96)
+		BFVM_BUCKET(112)
+		BFVM_BUCKET(128)
+		BFVM_BUCKET(144)
+		BFVM_BUCKET(160)
+		BFVM_BUCKET(176)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

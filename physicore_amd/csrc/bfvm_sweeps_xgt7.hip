@@ -1,0 +1,28 @@
+// Instantiations of the two-lane chunked x sweep for 7 substrates, rows of
+// 193..384 voxels (half-row buckets).
+#include "bfvm_xgt_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_xgt7(int nmax)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return sweep_xgt_kernel<N, 7>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(112)
+		BFVM_BUCKET(128)
+		BFVM_BUCKET(144)
+		BFVM_BUCKET(160)
+		BFVM_BUCKET(176)
+		BFVM_BUCKET(192)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

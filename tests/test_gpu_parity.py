@@ -105,6 +105,8 @@ CHUNK_SHAPES = [
     (3, (176, 70, 1), 1), (3, (177, 70, 1), 1), (3, (33, 65, 2), 1), (2, (100, 64, 1), 1),
     (3, (200, 33, 2), 2), (3, (17, 40, 1), 2), (2, (129, 31, 1), 2),
 ]
+CHUNK_SHAPES.remove((3, (177, 70, 1), 1))  # rows past one lane: the two-lane kernel (below)
+CHUNK_SHAPES.remove((3, (200, 5, 2), 7))
 
 
 @pytest.mark.parametrize("dims,grid,S", CHUNK_SHAPES)
@@ -122,6 +124,45 @@ def test_chunked_x_sweep_matches_strided(dims, grid, S):
     b.set_tuning(x_sweep=5)
     a.step(3)
     b.step(3)
+    assert np.array_equal(a.field(), b.field())
+
+
+# rows longer than one lane holds (past 176/192/200 voxels for S = 1/5,7/others):
+# the two-lane chunked kernel, its bucket edges, partial wave pairs, and the
+# longest rows of each S
+LONG_X_SHAPES = [
+    (3, (400, 40, 1), 2), (3, (201, 70, 2), 1), (3, (384, 12, 3), 3), (3, (230, 13, 2), 5),
+    (3, (390, 11, 1), 6), (3, (193, 9, 2), 7), (2, (300, 64, 1), 2), (3, (399, 33, 1), 2),
+    (3, (352, 65, 1), 1), (3, (384, 9, 1), 7), (3, (209, 21, 1), 3), (3, (385, 10, 2), 6),
+]
+
+
+@pytest.mark.parametrize("dims,grid,S", LONG_X_SHAPES)
+def test_two_lane_x_sweep_matches_oracle(dims, grid, S):
+    rng = np.random.default_rng(hash((dims, grid, S, 7)) % 2**32)
+    D = 10 ** rng.uniform(0, 5, S)
+    lam = rng.uniform(0.01, 1, S)
+    p = orc.Problem(dims, grid, (20, 20, 20), 0.01, D, lam)
+    f0 = rng.uniform(0, 1, p.shape)
+    want = p.diffuse(f0.astype(np.float32).astype(np.float64))
+    d = gpu_domain(p, field=f0)
+    d.set_tuning(x_sweep=5)
+    d.step(1)
+    err = rel_linf(d.field(), want)
+    assert err.max() < 2e-6, err
+
+
+def test_two_lane_x_sweep_batched_equals_alone():
+    from physicore_amd import bfvm
+    rng = np.random.default_rng(8)
+    p = orc.Problem(3, (400, 40, 3), (20, 20, 20), 0.01, [1e4, 30.0], [0.1, 0.5])
+    f0 = rng.uniform(0, 1, p.shape)
+    a, b = gpu_domain(p, field=f0), gpu_domain(p, field=f0)
+    for d in (a, b):
+        d.set_tuning(x_sweep=5)
+    batch = bfvm.Batch([b])
+    a.step(4)
+    batch.step(4)
     assert np.array_equal(a.field(), b.field())
 
 
