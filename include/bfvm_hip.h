@@ -221,8 +221,9 @@ typedef enum bfvm_x_sweep
 						   * LDS transpose of 32-voxel chunks (FP64): 16-byte accesses
 						   * for S in {4, 8}, 4-byte accesses otherwise (S <= 7); rows
 						   * of up to 200 voxels (176 for S = 1, 192 for S = 5, 7) and
-						   * at least one wave of rows (64/S, even for odd S); other
-						   * shapes use the strided lines */
+						   * at least one wave of rows (64/S, even for odd S).  Rows up
+						   * to twice that (S <= 7 but 4) take two lanes per line.
+						   * Other shapes use the strided lines or LDS tiles */
 } bfvm_x_sweep;
 
 /* Kernel selection and rounding.  persistent_wgs_per_cu: 0 = the default per
