@@ -49,7 +49,11 @@ constexpr int kPipeMinLine = 192;
 // not fit, 300 x 128^2 x 7: 507 (LDS tiles) / - / 101.
 int default_x_sweep(int nx, int S)
 {
-	if (S == 2 && nx >= 160 && bfvm::xg_bucket(nx, S))
+	// S = 2: the 16-byte chunked kernel for even rows below 176 voxels, the
+	// 4-byte one from 176 on (scan or transpose / chunked, us): 128^3: 26.1 /
+	// 22.0; 64 x 128^2: 18.8 / 15.5; 32 x 256^2: 25.9 / 19.6; 96 x 160^2: 32.8 /
+	// 22.2; 200^3: 48.9 / 37.0 (4-byte; 46.0 with 16-byte accesses)
+	if (S == 2 && nx <= 200 && (nx % 2 == 0 || nx >= 160) && (bfvm::xt_bucket(nx) || bfvm::xg_bucket(nx, S)))
 		return BFVM_X_CHUNKED;
 	if (nx > 200 || (S != 4 && S != 8 && nx > bfvm::xg_max_n(S)))
 		return bfvm::xscan_vpl(nx, S) ? BFVM_X_SCAN : BFVM_X_CHUNKED;
@@ -484,8 +488,9 @@ int build_tiles(bfvm_ctx* c)
 			}
 			// the scan and chunked kernels do not apply faces, which 1-D domains need in this sweep
 			// (the chunked kernel also needs a full wave of rows, 64/S)
-			else if (axis == 0 && xs == BFVM_X_CHUNKED && xt_bucket(n) && c->fp64 && (c->S == 4 || c->S == 8) &&
-					 c->n[1] * c->n[2] >= 64 / c->S && !(c->dims == 1 && c->face_mask))
+			else if (axis == 0 && xs == BFVM_X_CHUNKED && xt_bucket(n) && c->fp64 &&
+					 (c->S == 4 || c->S == 8 || (c->S == 2 && n % 2 == 0 && n < 176)) && c->n[1] * c->n[2] >= 64 / c->S &&
+					 !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepXT;
 				c->nmax[axis] = xt_bucket(n);

@@ -22,6 +22,8 @@ kernel_fn pick_cols_x32(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
 kernel_fn pick_rows(int nmax, bool fp64);
+kernel_fn pick_xt2a(int nmax);
+kernel_fn pick_xt2b(int nmax);
 kernel_fn pick_xt4a(int nmax);
 kernel_fn pick_xt4b(int nmax);
 kernel_fn pick_xt8a(int nmax);
@@ -128,9 +130,11 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 		return axis == 0 ? pick_rows(nmax, fp64) : nullptr;
 	if (variant == kSweepXT)
 	{
-		if (axis != 0 || !fp64 || (S != 4 && S != 8))
+		if (axis != 0 || !fp64 || (S != 2 && S != 4 && S != 8))
 			return nullptr;
 		const bool lo = nmax <= 96;
+		if (S == 2)
+			return lo ? pick_xt2a(nmax) : pick_xt2b(nmax);
 		return S == 4 ? (lo ? pick_xt4a(nmax) : pick_xt4b(nmax)) : (lo ? pick_xt8a(nmax) : pick_xt8b(nmax));
 	}
 	if (variant == kSweepXGT)
@@ -305,7 +309,7 @@ hipError_t init_sweep_kernels()
 			if (e != hipSuccess)
 				return e;
 		}
-	for (int S : { 4, 8 })
+	for (int S : { 2, 4, 8 })
 		for (int nmax = 8; nmax <= kXtMaxN; nmax += 8)
 		{
 			if (xt_bucket(nmax) != nmax)

@@ -1,6 +1,7 @@
 // bfvm_xt_kernels.h -- x sweep of rows of up to 200 voxels with 4 or 8
-// substrates: register lines filled and emptied through a per-wave LDS
-// transpose (device template; instantiated by bfvm_sweeps_xt{4,8}{a,b}.hip).
+// substrates, or 2 when rows have an even length (16-byte aligned rows):
+// register lines filled and emptied through a per-wave LDS transpose (device
+// template; instantiated by bfvm_sweeps_xt{2,4,8}{a,b}.hip).
 //
 // The strided register lines (sweep_cols_kernel<0, ...>) read each line with
 // stride S: one load instruction of a wave touches 64/S rows, 16 cache lines
@@ -38,7 +39,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xt_kernel(const DomainDev
 																 int finish, int fac_cap)
 {
 	(void)finish;
-	static_assert(S == 4 || S == 8, "16-byte transposes of 4 or 8 substrates");
+	static_assert(S == 2 || S == 4 || S == 8, "16-byte transposes of 2 (even rows), 4 or 8 substrates");
 	static_assert(NMAX % 8 == 0 && NMAX % 32 != 24 && NMAX <= kXtMaxN, "row bucket");
 	constexpr int R = 64 / S;	   // rows per wave
 	constexpr int PITCH = 33 * S;  // floats per row in the LDS buffer
