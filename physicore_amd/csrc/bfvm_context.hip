@@ -243,6 +243,7 @@ struct bfvm_ctx
 
 	DomainDev hdom {};
 	dvec<DomainDev> ddom;
+	dvec<DomainDev> dslots; // the captured graph's steps: ddom with step_offset 0, 1, ...
 
 	dvec<double> mass_partial, mass_out;
 
@@ -581,45 +582,50 @@ int sweep_grid(const bfvm_ctx* c, int axis)
 
 // One step on the stream: sweeps x, y, z (faces fused into the last), interior
 // Dirichlet, bulk, cells.  Stage order of openmp_solver.cpp:27-37.  `ev` (7
-// events or null) brackets the stages for timing.
-int enqueue_step(bfvm_ctx* c, hipEvent_t* ev)
+// events or null) brackets the stages for timing.  `dd` is the domain table
+// of the step (its step_offset places it in a captured run of steps), and
+// `tick` the steps the step count advances by at its end (0: none).
+int enqueue_step(bfvm_ctx* c, hipEvent_t* ev, const DomainDev* dd = nullptr, uint32_t tick = 1)
 {
+	if (!dd)
+		dd = c->ddom.p;
 	auto mark = [&](int k) -> hipError_t { return ev ? hipEventRecord(ev[k], c->stream) : hipSuccess; };
 	TRY_HIP(c, mark(0));
 	for (int axis = 0; axis < 3; axis++)
 	{
 		if (axis < c->dims && c->variant[axis] == kSweepPipe)
-			TRY_HIP(c, launch_sweep_pipe(axis, c->nmax[axis], axis == c->dims - 1 && c->face_mask != 0, c->ddom.p,
+			TRY_HIP(c, launch_sweep_pipe(axis, c->nmax[axis], axis == c->dims - 1 && c->face_mask != 0, dd,
 										 c->tiles[axis].p, c->ntiles[axis], 1, c->fac_cap[axis], c->pipe[axis],
 										 sweep_grid(c, axis), c->stream));
 		else if (axis < c->dims)
 			TRY_HIP(c, launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64,
-									axis == c->dims - 1 && c->face_mask != 0, c->S, c->ddom.p, c->tiles[axis].p,
+									axis == c->dims - 1 && c->face_mask != 0, c->S, dd, c->tiles[axis].p,
 									c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
 									c->stream));
 		TRY_HIP(c, mark(axis + 1));
 	}
 	if (c->dir_count)
-		TRY_HIP(c, launch_dirichlet_interior(c->ddom.p, c->dir_refs.p, c->dir_count, c->stream));
+		TRY_HIP(c, launch_dirichlet_interior(dd, c->dir_refs.p, c->dir_count, c->stream));
 	TRY_HIP(c, mark(4));
 	// bulk and cell apply as one launch when both run (voxels in both lists
 	// are applied in order by the bulk side); the stage timer then counts the
 	// cell apply with bulk
 	const bool both = c->bulk_count && c->n_bulk_tiles > 0 && c->n_owners > 0 && c->n_cell_tiles > 0 && c->overlap;
 	if (both)
-		TRY_HIP(c, launch_stages(c->ddom.p, c->bulk_tiles.p, c->n_bulk_tiles, c->cell_tiles.p, c->n_cell_tiles,
+		TRY_HIP(c, launch_stages(dd, c->bulk_tiles.p, c->n_bulk_tiles, c->cell_tiles.p, c->n_cell_tiles,
 								 c->stream));
 	else if (c->bulk_count)
-		TRY_HIP(c, launch_bulk(c->ddom.p, c->bulk_tiles.p, c->n_bulk_tiles, c->stream));
+		TRY_HIP(c, launch_bulk(dd, c->bulk_tiles.p, c->n_bulk_tiles, c->stream));
 	TRY_HIP(c, mark(5));
 	if (c->n_owners > 0)
 	{
 		if (!both)
-			TRY_HIP(c, launch_cells_apply(c->ddom.p, c->cell_tiles.p, c->n_cell_tiles, c->stream));
+			TRY_HIP(c, launch_cells_apply(dd, c->cell_tiles.p, c->n_cell_tiles, c->stream));
 		if (c->internalized && c->conflict)
-			TRY_HIP(c, launch_cells_internalized(c->ddom.p, c->int_tiles.p, c->n_int_tiles, c->stream));
+			TRY_HIP(c, launch_cells_internalized(dd, c->int_tiles.p, c->n_int_tiles, c->stream));
 	}
-	TRY_HIP(c, launch_step_tick(c->ddom.p, 1, c->stream));
+	if (tick)
+		TRY_HIP(c, launch_step_tick(dd, 1, tick, c->stream));
 	TRY_HIP(c, mark(6));
 	return BFVM_OK;
 }
@@ -855,6 +861,7 @@ struct bfvm_batch
 		int fac_cap = 0, grid = 0, ntiles = 0;
 		size_t lds = 0;
 		int64_t units = 0; // voxel*substrates swept: the launch's share of the axis
+		int lane = 0;	   // stream of the launch among the axis' lanes
 		PipeGeometry pg;
 		std::vector<int> doms;
 		dvec<TileRef> tiles;
@@ -870,9 +877,19 @@ struct bfvm_batch
 		dvec<TileRef> tiles;
 	};
 	std::vector<std::unique_ptr<SweepLaunch>> sweeps;
-	std::unique_ptr<RefLaunch> dirichlet;
-	std::unique_ptr<TileLaunch> bulk, cells, internalized;
+	// the stages after the diffusion of the domains whose last sweep runs on
+	// one lane of one axis, queued on that lane behind its sweeps: they
+	// overlap the sweeps of the other lanes
+	struct StageSet
+	{
+		std::unique_ptr<RefLaunch> dirichlet;
+		std::unique_ptr<TileLaunch> bulk, cells, internalized;
+		int launches() const { return (dirichlet ? 1 : 0) + (bulk ? 1 : 0) + (cells ? 1 : 0) + (internalized ? 1 : 0); }
+	};
+	StageSet stages[3][kBatchLanes];
 	dvec<DomainDev> ddoms;
+	std::vector<DomainDev> hdoms;
+	dvec<DomainDev> dslots; // the captured graph's steps: ddoms with step_offset 0, 1, ...
 	std::vector<uint64_t> signature;
 
 	hipGraph_t graph = nullptr;
@@ -968,6 +985,7 @@ int batch_prepare(bfvm_batch* b)
 		h[i] = b->ctx[i]->hdom;
 	if (hipError_t e = b->ddoms.upload(h.data(), h.size(), b->stream); e != hipSuccess)
 		return b->hip_fail(e, "domain table upload");
+	b->hdoms = h;
 
 	// sweeps: group domains by kernel
 	b->sweeps.clear();
@@ -1061,29 +1079,75 @@ int batch_prepare(bfvm_batch* b)
 		}
 	}
 
-	// boundary: refs (domain, entry); bulk and cells: sparse tiles of every
-	// domain, one launch each
+	// lanes: per axis, the biggest launches first, each on the lane with the
+	// least work so far, so that the lanes finish the axis together.  A
+	// launch's work is the data it sweeps (the sweeps are memory-bound) plus
+	// the stages of the domains whose last sweep it holds, which follow it on
+	// its lane (a sparse entry costs about 10 swept voxel*substrates)
+	constexpr int K = bfvm_batch::kBatchLanes;
+	auto stage_work = [&](int i) -> int64_t {
+		const bfvm_ctx* c = b->ctx[i];
+		const int64_t entries = c->dir_count + c->bulk_count + c->n_owners +
+								(c->internalized && c->conflict ? c->n_agents : 0);
+		return 10 * entries * c->S;
+	};
+	std::vector<int64_t> work(b->sweeps.size());
+	for (size_t j = 0; j < b->sweeps.size(); j++)
 	{
-		std::vector<SparseRef> v;
-		for (size_t i = 0; i < b->ctx.size(); i++)
-			for (int64_t k = 0; k < b->ctx[i]->dir_count; k++)
-				v.push_back({ (int32_t)i, (int32_t)k });
-		b->dirichlet.reset();
-		if (!v.empty())
-		{
-			b->dirichlet = std::make_unique<bfvm_batch::RefLaunch>();
-			b->dirichlet->n = (int64_t)v.size();
-			if (hipError_t e = b->dirichlet->refs.upload(v.data(), v.size(), b->stream); e != hipSuccess)
-				return b->hip_fail(e, "ref upload");
-		}
+		const auto& L = b->sweeps[j];
+		work[j] = L->units;
+		for (int i : L->doms)
+			if (L->axis == b->ctx[i]->dims - 1)
+				work[j] += stage_work(i);
 	}
-	auto make_tiles = [&](auto count_of, std::unique_ptr<bfvm_batch::TileLaunch>& out) -> int {
+	{
+		std::vector<size_t> order(b->sweeps.size());
+		for (size_t j = 0; j < order.size(); j++)
+			order[j] = j;
+		std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+			return b->sweeps[x]->axis != b->sweeps[y]->axis ? b->sweeps[x]->axis < b->sweeps[y]->axis
+															: work[x] > work[y];
+		});
+		std::vector<std::unique_ptr<bfvm_batch::SweepLaunch>> sorted;
+		for (size_t j : order)
+			sorted.push_back(std::move(b->sweeps[j]));
+		b->sweeps = std::move(sorted);
+		std::vector<int64_t> w2(order.size());
+		for (size_t j = 0; j < order.size(); j++)
+			w2[j] = work[order[j]];
+		work = std::move(w2);
+	}
+	for (int axis = 0; axis < 3; axis++)
+	{
+		int count = 0;
+		for (auto& L : b->sweeps)
+			count += L->axis == axis;
+		const int lanes = std::min(K, count);
+		int64_t load[K] = {};
+		for (size_t j = 0; j < b->sweeps.size(); j++)
+			if (b->sweeps[j]->axis == axis)
+			{
+				const int k = (int)(std::min_element(load, load + lanes) - load);
+				load[k] += work[j];
+				b->sweeps[j]->lane = k;
+			}
+	}
+
+	// stages per (last axis, lane): boundary refs (domain, entry); bulk,
+	// cells and internalized: sparse tiles of each domain
+	std::vector<int> lane_of(b->ctx.size(), 0);
+	for (auto& L : b->sweeps)
+		for (int i : L->doms)
+			if (L->axis == b->ctx[i]->dims - 1)
+				lane_of[i] = L->lane;
+	auto make_tiles = [&](int axis, int lane, auto count_of, std::unique_ptr<bfvm_batch::TileLaunch>& out) -> int {
 		std::vector<TileRef> t;
 		for (size_t i = 0; i < b->ctx.size(); i++)
-		{
-			auto ti = sparse_tiles((int)i, count_of(b->ctx[i]));
-			t.insert(t.end(), ti.begin(), ti.end());
-		}
+			if (b->ctx[i]->dims - 1 == axis && lane_of[i] == lane)
+			{
+				auto ti = sparse_tiles((int)i, count_of(b->ctx[i]));
+				t.insert(t.end(), ti.begin(), ti.end());
+			}
 		out.reset();
 		if (t.empty())
 			return BFVM_OK;
@@ -1093,35 +1157,74 @@ int batch_prepare(bfvm_batch* b)
 			return b->hip_fail(e, "tile upload");
 		return BFVM_OK;
 	};
-	int rc = make_tiles([](const bfvm_ctx* c) { return c->bulk_count; }, b->bulk);
-	if (!rc)
-		rc = make_tiles([](const bfvm_ctx* c) { return c->n_owners; }, b->cells);
-	if (!rc)
-		rc = make_tiles([](const bfvm_ctx* c) { return c->internalized && c->conflict ? c->n_agents : 0; },
-						b->internalized);
-	if (rc)
-		return rc;
+	for (int axis = 0; axis < 3; axis++)
+		for (int lane = 0; lane < K; lane++)
+		{
+			bfvm_batch::StageSet& st = b->stages[axis][lane];
+			std::vector<SparseRef> v;
+			for (size_t i = 0; i < b->ctx.size(); i++)
+				if (b->ctx[i]->dims - 1 == axis && lane_of[i] == lane)
+					for (int64_t k = 0; k < b->ctx[i]->dir_count; k++)
+						v.push_back({ (int32_t)i, (int32_t)k });
+			st.dirichlet.reset();
+			if (!v.empty())
+			{
+				st.dirichlet = std::make_unique<bfvm_batch::RefLaunch>();
+				st.dirichlet->n = (int64_t)v.size();
+				if (hipError_t e = st.dirichlet->refs.upload(v.data(), v.size(), b->stream); e != hipSuccess)
+					return b->hip_fail(e, "ref upload");
+			}
+			int rc = make_tiles(axis, lane, [](const bfvm_ctx* c) { return c->bulk_count; }, st.bulk);
+			if (!rc)
+				rc = make_tiles(axis, lane, [](const bfvm_ctx* c) { return c->n_owners; }, st.cells);
+			if (!rc)
+				rc = make_tiles(axis, lane,
+								[](const bfvm_ctx* c) { return c->internalized && c->conflict ? c->n_agents : 0; },
+								st.internalized);
+			if (rc)
+				return rc;
+		}
 	b->signature = sig;
 	return BFVM_OK;
 }
 
-int batch_enqueue_step(bfvm_batch* b)
+int batch_enqueue_stages(bfvm_batch* b, const DomainDev* dd, const bfvm_batch::StageSet& st, hipStream_t stream)
 {
-	constexpr int K = bfvm_batch::kBatchLanes;
+	if (st.dirichlet)
+		if (hipError_t e = launch_dirichlet_interior(dd, st.dirichlet->refs.p, st.dirichlet->n, stream);
+			e != hipSuccess)
+			return b->hip_fail(e, "dirichlet launch");
+	if (st.bulk)
+		if (hipError_t e = launch_bulk(dd, st.bulk->tiles.p, st.bulk->n, stream); e != hipSuccess)
+			return b->hip_fail(e, "bulk launch");
+	if (st.cells)
+		if (hipError_t e = launch_cells_apply(dd, st.cells->tiles.p, st.cells->n, stream); e != hipSuccess)
+			return b->hip_fail(e, "cells launch");
+	if (st.internalized)
+		if (hipError_t e = launch_cells_internalized(dd, st.internalized->tiles.p, st.internalized->n, stream);
+			e != hipSuccess)
+			return b->hip_fail(e, "internalized launch");
+	return BFVM_OK;
+}
+
+// One step of every domain: per axis, the groups on their lanes (forked from
+// and joined back into the batch stream), each lane followed by the stages of
+// the domains whose last sweep it ran; then the step counts advance by `tick`
+// (0: none).  `dd` is the domain table of the step (its step_offset places it
+// in a captured run of steps).
+int batch_enqueue_step(bfvm_batch* b, const DomainDev* dd = nullptr, uint32_t tick = 1)
+{
+	if (!dd)
+		dd = b->ddoms.p;
 	for (int axis = 0; axis < 3; axis++)
 	{
-		std::vector<bfvm_batch::SweepLaunch*> g;
+		int lanes = 0;
 		for (auto& L : b->sweeps)
 			if (L->axis == axis)
-				g.push_back(L.get());
-		if (g.empty())
+				lanes = std::max(lanes, L->lane + 1);
+		if (!lanes)
 			continue;
-		// biggest launches first (by the data they sweep: the sweeps are
-		// memory-bound), each on the stream with the least work so far, so
-		// that the streams finish the axis together
-		std::stable_sort(g.begin(), g.end(), [](auto* a, auto* c) { return a->units > c->units; });
-		const int lanes = std::min<int>(K, (int)g.size());
-		int64_t load[K] = {};
+		auto lane_stream = [&](int k) { return k == 0 ? b->stream : b->lane[k - 1]; };
 		if (lanes > 1)
 		{
 			if (hipError_t e = hipEventRecord(b->ev_fork, b->stream); e != hipSuccess)
@@ -1130,22 +1233,25 @@ int batch_enqueue_step(bfvm_batch* b)
 				if (hipError_t e = hipStreamWaitEvent(b->lane[k], b->ev_fork, 0); e != hipSuccess)
 					return b->hip_fail(e, "hipStreamWaitEvent");
 		}
-		for (size_t j = 0; j < g.size(); j++)
+		for (auto& Lp : b->sweeps)
 		{
-			bfvm_batch::SweepLaunch* L = g[j];
-			const int k = (int)(std::min_element(load, load + lanes) - load);
-			load[k] += L->units;
-			hipStream_t st = k == 0 ? b->stream : b->lane[k - 1];
+			const bfvm_batch::SweepLaunch* L = Lp.get();
+			if (L->axis != axis)
+				continue;
+			hipStream_t st = lane_stream(L->lane);
 			hipError_t e;
 			if (L->variant == kSweepPipe)
-				e = launch_sweep_pipe(L->axis, L->nmax, L->faces, b->ddoms.p, L->tiles.p, L->ntiles, 1, L->fac_cap,
+				e = launch_sweep_pipe(L->axis, L->nmax, L->faces, dd, L->tiles.p, L->ntiles, 1, L->fac_cap,
 									  L->pg, L->grid, st);
 			else
-				e = launch_sweep(L->axis, L->variant, L->nmax, L->fp64, L->faces, L->S, b->ddoms.p, L->tiles.p,
+				e = launch_sweep(L->axis, L->variant, L->nmax, L->fp64, L->faces, L->S, dd, L->tiles.p,
 								 L->ntiles, 1, L->fac_cap, L->lds, L->grid, st);
 			if (e != hipSuccess)
 				return b->hip_fail(e, "sweep launch");
 		}
+		for (int k = 0; k < lanes; k++)
+			if (int rc = batch_enqueue_stages(b, dd, b->stages[axis][k], lane_stream(k)))
+				return rc;
 		// the next axis starts when every group of this one is done
 		for (int k = 0; k < lanes - 1; k++)
 		{
@@ -1155,23 +1261,9 @@ int batch_enqueue_step(bfvm_batch* b)
 				return b->hip_fail(e, "hipStreamWaitEvent");
 		}
 	}
-	if (b->dirichlet)
-		if (hipError_t e = launch_dirichlet_interior(b->ddoms.p, b->dirichlet->refs.p, b->dirichlet->n, b->stream);
-			e != hipSuccess)
-			return b->hip_fail(e, "dirichlet launch");
-	if (b->bulk)
-		if (hipError_t e = launch_bulk(b->ddoms.p, b->bulk->tiles.p, b->bulk->n, b->stream); e != hipSuccess)
-			return b->hip_fail(e, "bulk launch");
-	if (b->cells)
-		if (hipError_t e = launch_cells_apply(b->ddoms.p, b->cells->tiles.p, b->cells->n, b->stream); e != hipSuccess)
-			return b->hip_fail(e, "cells launch");
-	if (b->internalized)
-		if (hipError_t e = launch_cells_internalized(b->ddoms.p, b->internalized->tiles.p, b->internalized->n,
-													 b->stream);
-			e != hipSuccess)
-			return b->hip_fail(e, "internalized launch");
-	if (hipError_t e = launch_step_tick(b->ddoms.p, (int)b->ctx.size(), b->stream); e != hipSuccess)
-		return b->hip_fail(e, "step tick launch");
+	if (tick)
+		if (hipError_t e = launch_step_tick(dd, (int)b->ctx.size(), tick, b->stream); e != hipSuccess)
+			return b->hip_fail(e, "step tick launch");
 	return BFVM_OK;
 }
 
@@ -1212,10 +1304,22 @@ int batch_run(bfvm_batch* b, int64_t iterations, int64_t graph_steps)
 		if (!b->graph_valid || b->graph_steps != graph_steps)
 		{
 			b->drop_graph();
+			// one domain table per captured step, the step counts advanced once
+			// at the end of the run
+			const size_t nd = b->hdoms.size();
+			std::vector<DomainDev> slots((size_t)graph_steps * nd);
+			for (int64_t k = 0; k < graph_steps; k++)
+				for (size_t i = 0; i < nd; i++)
+				{
+					slots[k * nd + i] = b->hdoms[i];
+					slots[k * nd + i].step_offset = (uint32_t)k;
+				}
+			if (hipError_t e = b->dslots.upload(slots.data(), slots.size(), b->stream); e != hipSuccess)
+				return b->hip_fail(e, "domain table upload");
 			if (hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal); e != hipSuccess)
 				return b->hip_fail(e, "hipStreamBeginCapture");
 			for (int64_t k = 0; k < graph_steps && rc == BFVM_OK; k++)
-				rc = batch_enqueue_step(b);
+				rc = batch_enqueue_step(b, b->dslots.p + k * nd, k == graph_steps - 1 ? (uint32_t)graph_steps : 0);
 			hipGraph_t g = nullptr;
 			hipError_t e = hipStreamEndCapture(b->stream, &g);
 			if (rc)
@@ -1591,10 +1695,16 @@ int bfvm_step_graph(bfvm_ctx* c, int64_t iterations, int64_t graph_steps)
 	if (!c->gexec || c->graph_version != c->version || c->graph_steps != graph_steps)
 	{
 		c->drop_graph();
+		// one domain table per captured step, the step count advanced once at
+		// the end of the run: no step-count update between the steps
+		std::vector<DomainDev> slots((size_t)graph_steps, c->hdom);
+		for (int64_t k = 0; k < graph_steps; k++)
+			slots[k].step_offset = (uint32_t)k;
+		TRY_HIP(c, c->dslots.upload(slots.data(), slots.size(), c->stream));
 		TRY_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
 		int rc = BFVM_OK;
 		for (int64_t k = 0; k < graph_steps && rc == BFVM_OK; k++)
-			rc = enqueue_step(c, nullptr);
+			rc = enqueue_step(c, nullptr, c->dslots.p + k, k == graph_steps - 1 ? (uint32_t)graph_steps : 0);
 		hipGraph_t g = nullptr;
 		hipError_t e = hipStreamEndCapture(c->stream, &g);
 		if (rc)
@@ -2014,8 +2124,13 @@ int bfvm_batch_info(bfvm_batch* b, int32_t* sweep_launches, int32_t* stage_launc
 	if (sweep_launches)
 		*sweep_launches = (int32_t)b->sweeps.size();
 	if (stage_launches)
-		*stage_launches = (int32_t)((b->dirichlet ? 1 : 0) + (b->bulk ? 1 : 0) + (b->cells ? 1 : 0) +
-									(b->internalized ? 1 : 0));
+	{
+		int n = 0;
+		for (const auto& axis : b->stages)
+			for (const auto& st : axis)
+				n += st.launches();
+		*stage_launches = (int32_t)n;
+	}
 	return BFVM_OK;
 }
 

@@ -21,6 +21,7 @@ struct DomainDev
 	int32_t nx, ny, nz, S, dims, L; // L = nx*S, one row
 	int32_t face_mask;				// bit f: face f has a condition for some substrate
 	int32_t round_nearest;			// 1: FP64 results rounded to nearest, not stochastically
+	uint32_t step_offset;			// this step's place in a run of steps captured as one graph
 	double dt, voxel_volume;
 	int64_t bbox_min[3];
 	int64_t voxel[3];
@@ -128,7 +129,7 @@ __host__ __device__ __forceinline__ uint32_t mix32(uint32_t x)
 
 __device__ __forceinline__ uint32_t stage_seed(const DomainDev& d, uint32_t stage)
 {
-	return mix32(*d.step * 8u + stage);
+	return mix32((*d.step + d.step_offset) * 8u + stage);
 }
 
 // FP64 -> FP32 with stochastic rounding, exact in any FP32 rounding mode: the
@@ -317,8 +318,9 @@ hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* ref
 // Sparse stages over tiles (dom, first entry, count <= kSparseTile).
 constexpr int kSparseTile = 256;
 hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
-// Ends a step of `ndoms` domains: advances each one's step count.
-hipError_t launch_step_tick(const DomainDev* doms, int ndoms, hipStream_t stream);
+// Ends a step, or a captured run of `by` steps, of `ndoms` domains: advances
+// each one's step count by `by`.
+hipError_t launch_step_tick(const DomainDev* doms, int ndoms, uint32_t by, hipStream_t stream);
 hipError_t launch_cells_apply(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 // Bulk and cell apply of one domain in one launch (bulk tiles first), once its

@@ -57,12 +57,13 @@ __global__ void bulk_kernel(const DomainDev* __restrict__ doms, const TileRef* _
 	}
 }
 
-// Advances the step count of every domain of a launch (the last kernel of a step).
-__global__ void step_tick_kernel(const DomainDev* __restrict__ doms, int ndoms)
+// Advances the step count of every domain of a launch by `by` (the last kernel
+// of a step, or of a captured run of `by` steps).
+__global__ void step_tick_kernel(const DomainDev* __restrict__ doms, int ndoms, uint32_t by)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < ndoms)
-		doms[i].step[0] += 1u;
+		doms[i].step[0] += by;
 }
 
 // ---------------------------------------------------------------- cells
@@ -716,11 +717,11 @@ hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, 
 	return hipGetLastError();
 }
 
-hipError_t launch_step_tick(const DomainDev* doms, int ndoms, hipStream_t stream)
+hipError_t launch_step_tick(const DomainDev* doms, int ndoms, uint32_t by, hipStream_t stream)
 {
 	if (ndoms <= 0)
 		return hipSuccess;
-	hipLaunchKernelGGL(step_tick_kernel, dim3((ndoms + 63) / 64), dim3(64), 0, stream, doms, ndoms);
+	hipLaunchKernelGGL(step_tick_kernel, dim3((ndoms + 63) / 64), dim3(64), 0, stream, doms, ndoms, by);
 	return hipGetLastError();
 }
 
