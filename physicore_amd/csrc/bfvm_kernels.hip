@@ -309,8 +309,8 @@ __global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_ow
 // compute_result, owner part (cell_solver.cpp:187-227, compute_densities :146-163,
 // compute_fused :165-185): D = (D + sum num) / sum den + sum fac at each occupied
 // voxel.  Without a conflict and with tracking on, the owner's internalized
-// amount takes V*(D_old - D_new); otherwise D_new is kept in FP64 for the
-// per-agent update.  One thread per (occupied voxel, substrate).  Agent
+// amount takes V*(D_old - D_new); with tracking on and a conflict, D_new is
+// kept in FP64 for the per-agent update (without tracking nothing is kept).  One thread per (occupied voxel, substrate).  Agent
 // activity is fixed between recomputes (new agent arrays always trigger one),
 // so every listed owner is active.
 // The cell update of apply entry k, substrate s, from the density d0 at its
@@ -324,8 +324,8 @@ __device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, d
 	const double d1 = (d0 + d.pnum[q]) / d.pden[q] + d.pfac[q];
 	if (fused)
 		d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
-	else
-		d.dnew[q] = d1;
+	else if (d.compute_internalized)
+		d.dnew[q] = d1; // read by the per-agent update (tracking on, shared voxels)
 	return sr_round(d1, mix32((uint32_t)e ^ seed), nearest);
 }
 
