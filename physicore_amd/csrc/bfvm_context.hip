@@ -311,12 +311,16 @@ struct bfvm_ctx
 
 namespace {
 
-// Tiles (dom, first, count <= kSparseTile) over `count` entries of one domain.
-std::vector<TileRef> sparse_tiles(int dom, int64_t count)
+// Tiles (dom, first, count) over `count` entries of one domain with S
+// substrates: at most kSparseTile / S entries a tile, so that each thread of a
+// 256-thread workgroup takes one (entry, substrate) and all of a tile's loads
+// are in flight at once.
+std::vector<TileRef> sparse_tiles(int dom, int64_t count, int S)
 {
+	const int per = std::max(1, kSparseTile / std::max(1, S));
 	std::vector<TileRef> t;
-	for (int64_t k = 0; k < count; k += kSparseTile)
-		t.push_back({ dom, (int32_t)k, (int32_t)std::min<int64_t>(kSparseTile, count - k), 0 });
+	for (int64_t k = 0; k < count; k += per)
+		t.push_back({ dom, (int32_t)k, (int32_t)std::min<int64_t>(per, count - k), 0 });
 	return t;
 }
 
@@ -726,7 +730,7 @@ int recompute(bfvm_ctx* c)
 	if (c->n_owners > 0)
 	{
 		TRY_HIP(c, launch_cells_pack(c->ddom.p, c->n_owners, s));
-		auto t = sparse_tiles(0, c->n_owners);
+		auto t = sparse_tiles(0, c->n_owners, c->S);
 		c->n_cell_tiles = (int)t.size();
 		TRY_HIP(c, c->cell_tiles.upload(t.data(), t.size(), s)); // synchronizes s
 		return rebuild_overlap(c);
@@ -1145,7 +1149,7 @@ int batch_prepare(bfvm_batch* b)
 		for (size_t i = 0; i < b->ctx.size(); i++)
 			if (b->ctx[i]->dims - 1 == axis && lane_of[i] == lane)
 			{
-				auto ti = sparse_tiles((int)i, count_of(b->ctx[i]));
+				auto ti = sparse_tiles((int)i, count_of(b->ctx[i]), b->ctx[i]->S);
 				t.insert(t.end(), ti.begin(), ti.end());
 			}
 		out.reset();
@@ -1581,7 +1585,7 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 	TRY_HIP(c, c->bulk_vox.upload(vox.data(), vox.size(), c->stream));
 	TRY_HIP(c, c->bulk_a.upload(a.data(), a.size(), c->stream));
 	TRY_HIP(c, c->bulk_b.upload(b.data(), b.size(), c->stream));
-	auto t = sparse_tiles(0, count);
+	auto t = sparse_tiles(0, count, c->S);
 	c->n_bulk_tiles = (int)t.size();
 	TRY_HIP(c, c->bulk_tiles.upload(t.data(), t.size(), c->stream));
 	if (count > 0)
@@ -1641,7 +1645,7 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->pvox.alloc(n));
 	TRY_HIP(c, c->cell_in_bulk.alloc(n));
 	TRY_HIP(c, c->slot_of.alloc(n));
-	auto t = sparse_tiles(0, n);
+	auto t = sparse_tiles(0, n, c->S);
 	c->n_int_tiles = (int)t.size();
 	TRY_HIP(c, c->int_tiles.upload(t.data(), t.size(), c->stream));
 	c->recompute_pending = true;

@@ -315,7 +315,8 @@ hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, 
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
 // The per-step sparse stages run one thread per (entry, substrate); `S` is the
 // substrate count of every referenced domain (batched launches group by S).
-// Sparse stages over tiles (dom, first entry, count <= kSparseTile).
+// Sparse stages over tiles (dom, first entry, count <= kSparseTile / S), one
+// 256-thread workgroup a tile.
 constexpr int kSparseTile = 256;
 hipError_t launch_bulk(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream);
 // Ends a step, or a captured run of `by` steps, of `ndoms` domains: advances

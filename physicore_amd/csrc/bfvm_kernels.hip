@@ -38,8 +38,8 @@ __global__ void dirichlet_interior_kernel(const DomainDev* __restrict__ doms, co
 // with a = dt*S*T and b = 1 + dt*(U+S) precomputed in FP64 on the host.  One
 // thread per (listed voxel, substrate).
 // Sparse stages (bulk, cell apply, per-agent internalized) run over tiles of
-// up to kSparseTile entries of one domain: tile = (dom, first entry, count), one
-// workgroup per tile, so the domain's descriptor is read once into SGPRs.
+// up to kSparseTile / S entries of one domain: tile = (dom, first entry, count),
+// one workgroup per tile, so the domain's descriptor is read once into SGPRs.
 __global__ void bulk_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
 	const TileRef tr = tiles[blockIdx.x];
