@@ -730,6 +730,7 @@ int recompute(bfvm_ctx* c)
 	if (c->n_owners > 0)
 	{
 		TRY_HIP(c, launch_cells_pack(c->ddom.p, c->n_owners, s));
+		TRY_HIP(c, launch_cells_agent_slots(c->ddom.p, n, s));
 		auto t = sparse_tiles(0, c->n_owners, c->S);
 		c->n_cell_tiles = (int)t.size();
 		TRY_HIP(c, c->cell_tiles.upload(t.data(), t.size(), s)); // synchronizes s

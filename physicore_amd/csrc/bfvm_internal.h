@@ -62,7 +62,7 @@ struct DomainDev
 	const uint8_t* is_active;
 	// cell stage state, rebuilt at recompute
 	int32_t* cell_voxel; // [n] linear voxel, -1 inactive
-	int32_t* owner_of;	 // [n] owner agent of the cell's voxel
+	int32_t* owner_of;	 // [n] owner agent of the cell's voxel; once the owner list is packed, the owner's slot in it
 	int32_t* ballot;	 // [V] owner agent per voxel (atomic mode)
 	double* num;		 // [n*S] per-agent terms
 	double* den;
@@ -347,6 +347,8 @@ hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stre
 hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream);
 // packs the apply list from owners[0..n_owners) (both modes)
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream);
+// After the pack: owner_of[i] becomes the slot of agent i's owner (-1 stays).
+hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream);
 // atomic mode: keys/vals = (voxel, owner) of the collected owner list, to sort
 // it into voxel order (radix_sort_pairs)
 hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n_owners,

@@ -306,6 +306,20 @@ __global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_ow
 	}
 }
 
+// Replaces each agent's owner by the owner's slot in the packed list, so that
+// the per-step per-agent update reads its owner's results with one gather.
+// Thread i reads and writes only owner_of[i]; slot_of is read only.
+__global__ void cells_agent_slots_kernel(const DomainDev* __restrict__ dp, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const DomainDev& d = *dp;
+	const int o = d.owner_of[i];
+	if (o >= 0)
+		d.owner_of[i] = d.slot_of[o];
+}
+
 // compute_result, owner part (cell_solver.cpp:187-227, compute_densities :146-163,
 // compute_fused :165-185): D = (D + sum num) / sum den + sum fac at each occupied
 // voxel.  Without a conflict and with tracking on, the owner's internalized
@@ -426,10 +440,10 @@ __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, co
 	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 	{
 		const int i = tr.a + t / S, s = t % S;
-		if (d.cell_voxel[i] < 0)
+		const int64_t slot = d.owner_of[i]; // the owner's slot (-1: inactive agent)
+		if (slot < 0)
 			continue;
 		const int64_t q = (int64_t)i * S + s;
-		const int64_t slot = d.slot_of[d.owner_of[i]];
 		d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[slot * S + s] * d.den[q] + d.fac[q]);
 	}
 }
@@ -776,6 +790,11 @@ hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream)
 {
 	return launch1d(cells_pack_kernel, n_owners, 256, stream, dom, n_owners);
+}
+
+hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_agent_slots_kernel, n, 256, stream, dom, n);
 }
 
 hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream)
