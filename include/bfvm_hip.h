@@ -206,6 +206,13 @@ int bfvm_stream(bfvm_ctx* ctx, void** stream);
  * (0 when a stage does not run). */
 int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 
+/* Kernel timing of one sweep: `launches` back-to-back launches of the sweep
+ * along `axis` (0 = x) between two HIP events on the context stream; *out_us
+ * = the average per launch, without the gaps between separately timed
+ * launches.  Measurement only: the field takes `launches` extra sweeps along
+ * that axis (no counterpart in the reference). */
+int bfvm_time_sweep(bfvm_ctx* ctx, int32_t axis, int64_t launches, double* out_us);
+
 /* Kernel of the x sweep.  A kernel that does not take the shape falls back to
  * the strided lines; rows longer than 256 voxels run on the scan kernel where
  * it fits (up to 512 voxels, S*ceil(nx/64) <= 32), else inside LDS tiles. */

@@ -76,6 +76,7 @@ _SIGS = {
     "bfvm_field_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "bfvm_stream": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "bfvm_time_stages": ([C.c_void_p, C.c_int64, f64p], C.c_int),
+    "bfvm_time_sweep": ([C.c_void_p, C.c_int32, C.c_int64, f64p], C.c_int),
     "bfvm_get_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
     "bfvm_set_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
     "bfvm_batch_create": ([C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
@@ -311,6 +312,13 @@ class Domain:
         out = np.zeros(6)
         self._check(lib().bfvm_time_stages(self._h, int(steps), _p(out)))
         return dict(zip(("x", "y", "z", "dirichlet", "bulk", "cells"), out.tolist()))
+
+    def time_sweep(self, axis, launches):
+        """Average duration in microseconds of `launches` back-to-back sweeps
+        along `axis` (measurement only: the field takes the extra sweeps)."""
+        out = np.zeros(1)
+        self._check(lib().bfvm_time_sweep(self._h, int(axis), int(launches), _p(out)))
+        return float(out[0])
 
     def tuning(self):
         t = Tuning()

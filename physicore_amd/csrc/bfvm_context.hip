@@ -584,6 +584,17 @@ int sweep_grid(const bfvm_ctx* c, int axis)
 	return c->ntiles[axis];
 }
 
+// The sweep of one axis on the context stream (faces fused into the last axis).
+hipError_t launch_axis_sweep(bfvm_ctx* c, int axis, const DomainDev* dd)
+{
+	const bool faces = axis == c->dims - 1 && c->face_mask != 0;
+	if (c->variant[axis] == kSweepPipe)
+		return launch_sweep_pipe(axis, c->nmax[axis], faces, dd, c->tiles[axis].p, c->ntiles[axis], 1,
+								 c->fac_cap[axis], c->pipe[axis], sweep_grid(c, axis), c->stream);
+	return launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64, faces, c->S, dd, c->tiles[axis].p,
+						c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis), c->stream);
+}
+
 // One step on the stream: sweeps x, y, z (faces fused into the last), interior
 // Dirichlet, bulk, cells.  Stage order of openmp_solver.cpp:27-37.  `ev` (7
 // events or null) brackets the stages for timing.  `dd` is the domain table
@@ -597,15 +608,8 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev, const DomainDev* dd = nullptr, uin
 	TRY_HIP(c, mark(0));
 	for (int axis = 0; axis < 3; axis++)
 	{
-		if (axis < c->dims && c->variant[axis] == kSweepPipe)
-			TRY_HIP(c, launch_sweep_pipe(axis, c->nmax[axis], axis == c->dims - 1 && c->face_mask != 0, dd,
-										 c->tiles[axis].p, c->ntiles[axis], 1, c->fac_cap[axis], c->pipe[axis],
-										 sweep_grid(c, axis), c->stream));
-		else if (axis < c->dims)
-			TRY_HIP(c, launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64,
-									axis == c->dims - 1 && c->face_mask != 0, c->S, dd, c->tiles[axis].p,
-									c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis),
-									c->stream));
+		if (axis < c->dims)
+			TRY_HIP(c, launch_axis_sweep(c, axis, dd));
 		TRY_HIP(c, mark(axis + 1));
 	}
 	if (c->dir_count)
@@ -1962,6 +1966,42 @@ int bfvm_stream(bfvm_ctx* c, void** stream)
 		return BFVM_ERR_INVALID;
 	FLUSH_RELEASES(c);
 	*stream = (void*)c->stream;
+	return BFVM_OK;
+}
+
+int bfvm_time_sweep(bfvm_ctx* c, int32_t axis, int64_t launches, double* out_us)
+{
+	if (!c || !out_us || launches < 1 || axis < 0 || axis >= c->dims)
+		return BFVM_ERR_INVALID;
+	FLUSH_RELEASES(c);
+	if (c->recompute_pending)
+	{
+		int rc = recompute(c);
+		if (rc)
+			return rc;
+	}
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	TRY_HIP(c, hipEventCreate(&e0));
+	if (hipError_t e = hipEventCreate(&e1); e != hipSuccess)
+	{
+		(void)hipEventDestroy(e0);
+		return c->hip_fail(e, "hipEventCreate");
+	}
+	hipError_t e = hipEventRecord(e0, c->stream);
+	for (int64_t k = 0; k < launches && e == hipSuccess; k++)
+		e = launch_axis_sweep(c, axis, c->ddom.p);
+	if (e == hipSuccess)
+		e = hipEventRecord(e1, c->stream);
+	if (e == hipSuccess)
+		e = hipEventSynchronize(e1);
+	float ms = 0;
+	if (e == hipSuccess)
+		e = hipEventElapsedTime(&ms, e0, e1);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (e != hipSuccess)
+		return c->hip_fail(e, "sweep timing");
+	*out_us = (double)ms * 1000.0 / (double)launches;
 	return BFVM_OK;
 }
 

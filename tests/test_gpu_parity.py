@@ -614,3 +614,21 @@ def test_config2_full_size_short():
     p, f0, dirs, bulk, ag = sc.config2(side=200, S=4, n_cells=100_000, seed=43)
     d, f_or, _ = _full_run(p, f0, dirs, bulk, ag, 3, internalized=False)
     assert rel_linf(d.field(), f_or).max() < 2e-6
+
+
+def test_time_sweep_reports_each_axis():
+    """bfvm_time_sweep: a positive average for every axis of the domain; an
+    axis beyond its dims is rejected."""
+    from physicore_amd import bfvm
+
+    require_gpu()
+    p = orc.Problem(3, (32, 24, 16), (20, 20, 20), 0.01, [100.0, 10.0], [0.1, 0.2])
+    d = gpu_domain(p, field=np.ones(p.shape))
+    for axis in range(3):
+        assert d.time_sweep(axis, 5) > 0
+    with pytest.raises(bfvm.BfvmError):
+        d.time_sweep(3, 1)
+    q = orc.Problem(2, (32, 24, 1), (20, 20, 20), 0.01, [100.0], [0.1])
+    e = gpu_domain(q, field=np.ones(q.shape))
+    with pytest.raises(bfvm.BfvmError):
+        e.time_sweep(2, 1)
