@@ -55,6 +55,11 @@ int default_x_sweep(int nx, int S)
 	// 22.2; 200^3: 48.9 / 37.0 (4-byte; 46.0 with 16-byte accesses)
 	if (S == 2 && nx <= 200 && (nx % 2 == 0 || nx >= 160) && (bfvm::xt_bucket(nx) || bfvm::xg_bucket(nx, S)))
 		return BFVM_X_CHUNKED;
+	// S = 2 past one-lane rows: the two-lane chunked kernel (scan / chunked,
+	// us, back-to-back launches): 400^2 x 100: 74.9 / 69.8; 300 x 200 x 100:
+	// 40.7 / 34.1
+	if (S == 2 && nx > 200 && bfvm::xgt_bucket(nx, S))
+		return BFVM_X_CHUNKED;
 	if (nx > 200 || (S != 4 && S != 8 && nx > bfvm::xg_max_n(S)))
 		return bfvm::xscan_vpl(nx, S) ? BFVM_X_SCAN : BFVM_X_CHUNKED;
 	if (S <= 2 || nx > bfvm::kRegMaxN)
