@@ -23,6 +23,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from physicore_amd import workloads as wl  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sweeps_200cube_s4_hbm_traffic.json")
 
@@ -41,114 +43,81 @@ def measured_traffic(kernel, side, S):
     return {"bytes": kib * 1024, "kernel_symbol": k["kernel"], "source": "profiles/" + prof["source"]}
 
 
-def build_c2(side, S, n_cells, seed):
-    """Config-2 inputs, generated without the oracle (same recipe as tests/scenarios.py)."""
-    rng = np.random.default_rng(seed)
-    dx = 20
-    box = side * dx
-    D = 10.0 ** rng.uniform(0.0, 5.0, S)
-    lam = rng.uniform(0.01, 1.0, S)
-    field = rng.uniform(0.0, 1.0, (side, side, side, S))
-    c = (np.arange(side) + 0.5) * dx - box / 2.0
-    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
-    inside = ((xx ** 2 + yy ** 2 + zz ** 2) <= 500.0 ** 2).reshape(-1)
-    bulk_vox = np.nonzero(inside)[0]
-    nb = len(bulk_vox)
-    bulk = (bulk_vox, np.full((nb, S), 0.5), np.zeros((nb, S)), np.ones((nb, S)))
-    pos = rng.uniform(0.0, box, (n_cells, 3))
-    np.minimum(pos, np.nextafter(box, 0), out=pos)
-    sec = rng.uniform(0.0, 10.0, (n_cells, S))
-    upt = rng.uniform(0.0, 10.0, (n_cells, S))
-    agents = dict(positions=pos, secretion=sec, saturation=np.ones((n_cells, S)), uptake=upt,
-                  net_export=np.zeros((n_cells, S)), internalized=None,
-                  fraction_released=np.zeros((n_cells, S)), volumes=np.full(n_cells, 2494.0),
-                  is_active=np.ones(n_cells, dtype=np.uint8))
-    return dict(side=side, S=S, D=D, lam=lam, field=field, bulk=bulk, agents=agents, dt=0.01, dx=dx)
+def _oracle_inputs(w):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenarios
+
+    return scenarios.oracle_inputs(w)
 
 
-def gpu_domain(w, precision="fp64"):
-    from physicore_amd import bfvm
-
-    side, S = w["side"], w["S"]
-    d = bfvm.Domain(3, (side, side, side), (w["dx"],) * 3, w["dt"], w["D"], w["lam"], np.zeros(S),
-                    precision=precision)
-    d.upload_field(w["field"])
-    vals = np.zeros((6, S))
-    conds = np.zeros((6, S), dtype=np.uint8)
-    vals[0], conds[0] = 1.0, 1
-    d.set_dirichlet_faces(vals, conds)
-    d.set_bulk(*w["bulk"])
-    a = w["agents"]
-    d.set_agents(a["positions"], a["secretion"], a["saturation"], a["uptake"], a["net_export"],
-                 a["internalized"], a["fraction_released"], a["volumes"], a["is_active"])
-    return d
-
-
-def cpu_baseline(w, budget_s=12.0):
-    """The CPU oracle (FP64 restatement of the OpenMP solver) on this host's cores,
-    on a bounded sample of the same workload: as many whole steps as fit ~budget_s."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    side, S = w["side"], w["S"]
-    p = orc.Problem(3, (side, side, side), (w["dx"],) * 3, w["dt"], w["D"], w["lam"])
-    field = w["field"].copy()
-    dirs = orc.DirichletSpec(S, 3)
-    dirs.set_face(0, "min", np.ones(S), np.ones(S, dtype=np.uint8))
-    vox, sup, upt, tgt = w["bulk"]
-    dense = [np.zeros(p.shape) for _ in range(3)]
-    for arr, src in zip(dense, (sup, upt, tgt)):
-        arr.reshape(-1, S)[vox] = src
-    a = w["agents"]
-    ag = orc.AgentSpec(len(a["volumes"]), S, 3)
-    for k in ("positions", "secretion", "saturation", "uptake", "net_export", "fraction_released",
-              "volumes", "is_active"):
-        setattr(ag, k, np.ascontiguousarray(a[k]))
-    s = orc.Solver(p, field, dirichlet=dirs, bulk=dense, agents=ag)
+    return orc
+
+
+def cpu_baseline(w, budget_s=12.0, recompute_every=0):
+    """The CPU oracle (FP64 restatement of the OpenMP solver) on this host's cores,
+    on a bounded sample of the same workload: as many whole steps as fit ~budget_s
+    (with a positional recompute every `recompute_every` steps when asked)."""
+    orc = _oracle()
+    p, field, dirs, bulk, ag = _oracle_inputs(w)
+    s = orc.Solver(p, field, dirichlet=dirs, bulk=bulk, agents=ag, compute_internalized=w.compute_internalized)
     s.solve(1)  # warm-up (includes the recompute), excluded like the reference benchmark
     steps, t0 = 0, time.perf_counter()
     while True:
+        if recompute_every and steps % recompute_every == 0:
+            s.recompute()
         s.solve(1)
         steps += 1
         el = time.perf_counter() - t0
         if el >= budget_s and steps >= 2:
             break
-    ups = side ** 3 * S * steps / el
+    ups = w.units * steps / el
+    rc = f", recompute every {recompute_every} steps" if recompute_every and "recompute" not in w.name else ""
     return {"value": ups, "unit": "voxel*substrate updates/s", "cores": orc.num_threads(),
             "kind": "port",
-            "sample": f"{steps} whole FP64 steps of the same {side}^3x{S} workload (incl. bulk, "
-                      f"cells, Dirichlet) in {el:.1f}s, after 1 warm-up step"}
+            "sample": f"{steps} whole FP64 steps of the same workload ({w.name}{rc}) in {el:.1f}s, "
+                      f"after 1 warm-up step"}
 
 
-def build_c3_domains(seed=44, count=64, sides=tuple(range(32, 161, 16)), max_S=8, max_cells=50_000):
-    """Batched ensemble: `count` domains, each axis side drawn from `sides`
-    (anisotropic), dx=20, S uniform in 1..max_S with D/decay as the single large
-    domain, field U[0,1], 0..max_cells uniform cells (secretion/uptake U[0,10],
-    saturation 1, volume 2494).  Fields are generated and uploaded one domain at
-    a time."""
-    from physicore_amd import bfvm
+def cpu_baseline_domains(workloads, steps_each=2):
+    """The CPU oracle over several domains one after another (the sequential CPU
+    form of a batch or a request stream): per domain one warm-up step, then
+    `steps_each` timed steps.  Returns the aggregate updates/s and the per-domain
+    seconds per step."""
+    orc = _oracle()
+    units = secs = 0.0
+    per_step = []
+    for w in workloads:
+        p, field, dirs, bulk, ag = _oracle_inputs(w)
+        s = orc.Solver(p, field, dirichlet=dirs, bulk=bulk, agents=ag,
+                       compute_internalized=w.compute_internalized)
+        s.solve(1)
+        t0 = time.perf_counter()
+        s.solve(steps_each)
+        el = time.perf_counter() - t0
+        units += w.units * steps_each
+        secs += el
+        per_step.append(el / steps_each)
+    return units / secs, per_step, orc.num_threads()
 
-    rng = np.random.default_rng(seed)
+
+def build_c3_domains(seed=44, count=64):
+    """The batched ensemble (physicore_amd.workloads.c3): `count` domains, each
+    axis side drawn from {32..160} (anisotropic), dx=20, S uniform in 1..8 with
+    D/decay as the single large domain, field U[0,1], 0..50k uniform cells.
+    Fields are generated and uploaded one domain at a time."""
     doms, units = [], 0
-    for _ in range(count):
-        n = [int(rng.choice(sides)) for _ in range(3)]
-        S = int(rng.integers(1, max_S + 1))
-        D = 10.0 ** rng.uniform(0.0, 5.0, S)
-        lam = rng.uniform(0.01, 1.0, S)
-        d = bfvm.Domain(3, tuple(n), (20, 20, 20), 0.01, D, lam, np.zeros(S))
-        d.upload_field(rng.uniform(0.0, 1.0, (n[2], n[1], n[0], S)).astype(np.float32))
-        k = int(rng.integers(0, max_cells + 1))
-        if k:
-            box = np.array(n, dtype=np.float64) * 20.0
-            pos = np.minimum(rng.uniform(0.0, 1.0, (k, 3)) * box, np.nextafter(box, 0))
-            d.set_agents(pos, rng.uniform(0, 10, (k, S)), np.ones((k, S)), rng.uniform(0, 10, (k, S)),
-                         np.zeros((k, S)), None, np.zeros((k, S)), np.full(k, 2494.0), np.ones(k, dtype=np.uint8))
-        doms.append(d)
-        units += n[0] * n[1] * n[2] * S
+    for w in wl.c3(seed, count):
+        doms.append(wl.make_domain(w))
+        units += w.units
     return doms, units
 
 
-def bench_batched(steps, graph_steps=10, seed=44):
+def bench_batched(steps, graph_steps=10, seed=44, cpu=True):
     """The 64-domain ensemble stepped as one batch (after autotuning) and, for
     comparison, the same domains stepped one after another on the plain path."""
     from physicore_amd import bfvm
@@ -177,7 +146,9 @@ def bench_batched(steps, graph_steps=10, seed=44):
     plain = units * steps / el_plain
     gbs = units * 24 / (el / steps) / 1e9
     b.close()
-    return {"workload": f"{len(doms)} domains, sides from {{32..160}} step 16 per axis, S 1-8, "
+    count = len(doms)
+    del doms
+    out = {"workload": f"{count} domains, sides from {{32..160}} step 16 per axis, S 1-8, "
                         f"0-50k cells each, seed {seed}",
             "voxel_substrates": units, "steps": steps, "graph_steps": graph_steps,
             "value": batched, "unit": "voxel*substrate updates/s",
@@ -187,9 +158,16 @@ def bench_batched(steps, graph_steps=10, seed=44):
             "launches_per_step": info, "tuning": tuning,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "bytes_per_step": units * 24}}
+    if cpu:
+        print("bench.py: CPU baseline of the ensemble", file=sys.stderr, flush=True)
+        ups, _, cores = cpu_baseline_domains(wl.c3(seed), steps_each=2)
+        out["cpu_baseline"] = {"value": ups, "unit": "voxel*substrate updates/s", "cores": cores, "kind": "port",
+                               "sample": "the FP64 oracle over all 64 domains one after another, "
+                                         "2 timed steps each after 1 warm-up step"}
+    return out
 
 
-def bench_served(count, rate, seed=46):
+def bench_served(count, rate, seed=46, cpu=True):
     """Served stream: Poisson arrivals of requests (cube side log-uniform in
     [32, 256], S 1-6, 0-200k cells, 100-5000 steps), HBM budget 200 GB,
     continuous batching in chunks of 10 steps."""
@@ -204,59 +182,67 @@ def bench_served(count, rate, seed=46):
     done = srv.run()
     wall = srv.now() - t0
     ttc = [r.finished - r.arrival for r in done]
-    return {"workload": f"{count} requests, Poisson rate {rate}/s, side log-uniform [32,256], S 1-6, "
+    out = {"workload": f"{count} requests, Poisson rate {rate}/s, side log-uniform [32,256], S 1-6, "
                         f"0-200k cells, 100-5000 steps, budget 200 GB, chunk 10, seed {seed}",
             "requests": len(done), "wall_s": wall,
             "value": srv.updates / wall, "unit": "voxel*substrate updates/s",
             "p50_s": serve.percentile(ttc, 50), "p99_s": serve.percentile(ttc, 99),
             "chunks": srv.chunks}
+    if cpu:
+        # the oracle per request, one after another: 2 timed steps each, scaled
+        # to the request's step count
+        print("bench.py: CPU baseline of the stream", file=sys.stderr, flush=True)
+        reqs = serve.poisson_stream(np.random.default_rng(seed), count, rate, random_field=False)
+        ups, per_step, cores = cpu_baseline_domains([serve.workload_of(r) for r in reqs], steps_each=2)
+        t_req = [t * r.steps for t, r in zip(per_step, reqs)]
+        out["cpu_baseline"] = {"value": ups, "unit": "voxel*substrate updates/s", "cores": cores, "kind": "port",
+                               "sequential_s": float(sum(t_req)),
+                               "p50_request_s": serve.percentile(t_req, 50),
+                               "p99_request_s": serve.percentile(t_req, 99),
+                               "sample": f"the FP64 oracle over all {count} requests one after another, 2 timed "
+                                         f"steps each after 1 warm-up step, scaled to each request's steps"}
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--side", type=int, default=200)
-    ap.add_argument("--substrates", type=int, default=4)
-    ap.add_argument("--cells", type=int, default=100_000)
-    ap.add_argument("--graph-steps", type=int, default=10)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--batched-steps", type=int, default=100)
-    ap.add_argument("--served-requests", type=int, default=32)
-    ap.add_argument("--served-rate", type=float, default=8.0)
-    ap.add_argument("--only-plain", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy a 256 MB device buffer 10 times (reference for HBM counters)")
-    args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
+def bench_reference(steps=100, cpu=True, budget_s=12.0):
+    """The reference's own benchmark workload (openmp_solver/examples/benchmark.cpp:
+    250^3 x 4, bulk in every voxel, six Dirichlet faces, 2e6+1 lattice agents,
+    internalized tracking), stepped with a positional recompute every 10 steps
+    as that program does, and without."""
+    w = wl.reference_benchmark(steps=steps)
+    d = wl.make_domain(w)
+    period = w.recompute_every
+    d.step_graph(period, period)
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps // period):
+        d.recompute_positional_data()
+        d.step_graph(period, period)
+    d.sync()
+    el_rc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d.step_graph(steps, period)
+    d.sync()
+    el = time.perf_counter() - t0
+    stages = d.time_stages(period)
+    out = {"workload": w.name, "steps": steps, "voxel_substrates": w.units,
+           "value": w.units * steps / el_rc, "unit": "voxel*substrate updates/s",
+           "ms_per_step_with_recompute": el_rc * 1e3 / steps,
+           "ms_per_step_without_recompute": el * 1e3 / steps,
+           "stages_us": stages}
+    d.close()
+    if cpu:
+        print("bench.py: CPU baseline of the reference workload", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, budget_s, recompute_every=period)
+    return out
 
-        src = torch.rand(64 * 1024 * 1024, device="cuda:0")  # 256 MB of FP32
-        dst = torch.empty_like(src)
-        for _ in range(10):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
-                          "launches": 10}), flush=True)
-        return
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
-              "(independent domains would spread over GPUs by HBM budget, one large domain by z-slabs)",
-              file=sys.stderr)
-        sys.exit(2)
 
-    from physicore_amd import bfvm
-
-    if bfvm.device_count() < 1:
-        print("bench.py: no HIP device", file=sys.stderr)
-        sys.exit(1)
-    w = build_c2(args.side, args.substrates, args.cells, seed=43)
-    d = gpu_domain(w, args.precision)
-    V = args.side ** 3
-    units = V * args.substrates
+def bench_plain(args):
+    """The single large domain (workloads.c2), whole steps replayed from
+    graphs; returns the workload and the headline fields of the JSON line."""
+    w = wl.c2(args.side, args.substrates, args.cells, seed=43)
+    d = wl.make_domain(w, precision=args.precision)
+    units = w.units
 
     d.step_graph(args.warmup, args.graph_steps)
     d.sync()
@@ -312,13 +298,69 @@ def main():
         "stages_us": stages,
         "sweeps_back_to_back_us": sweeps,
     }
-    del d
-    if not args.only_plain:
+    d.close()
+    return w, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--side", type=int, default=200)
+    ap.add_argument("--substrates", type=int, default=4)
+    ap.add_argument("--cells", type=int, default=100_000)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--served-requests", type=int, default=32)
+    ap.add_argument("--served-rate", type=float, default=8.0)
+    ap.add_argument("--only-plain", action="store_true", help="same as --legs plain")
+    ap.add_argument("--legs", default="plain,batched,served,reference",
+                    help="comma-separated parts to run: plain, batched, served, reference")
+    ap.add_argument("--reference-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MB device buffer 10 times (reference for HBM counters)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.rand(64 * 1024 * 1024, device="cuda:0")  # 256 MB of FP32
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                          "launches": 10}), flush=True)
+        return
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
+              "(independent domains would spread over GPUs by HBM budget, one large domain by z-slabs)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from physicore_amd import bfvm
+
+    if bfvm.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    legs = {"plain"} if args.only_plain else set(args.legs.split(","))
+    out = {}
+    if "plain" in legs:
+        w, out = bench_plain(args)
+    cpu = not args.no_cpu_baseline
+    if "batched" in legs:
         print("bench.py: batched ensemble", file=sys.stderr, flush=True)
-        out["batched"] = bench_batched(args.batched_steps)
+        out["batched"] = bench_batched(args.batched_steps, cpu=cpu)
+    if "served" in legs:
         print("bench.py: served stream", file=sys.stderr, flush=True)
-        out["served"] = bench_served(args.served_requests, args.served_rate)
-    if not args.no_cpu_baseline:
+        out["served"] = bench_served(args.served_requests, args.served_rate, cpu=cpu)
+    if "reference" in legs:
+        print("bench.py: reference benchmark workload", file=sys.stderr, flush=True)
+        out["reference_benchmark"] = bench_reference(args.reference_steps, cpu=cpu, budget_s=args.cpu_budget)
+    if cpu and "plain" in legs:
         print("bench.py: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)

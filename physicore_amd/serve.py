@@ -100,16 +100,7 @@ class Server:
         still = []
         for r in self.waiting:
             if r.arrival <= t and self.in_use + r.device_bytes() <= self.budget:
-                d = bfvm.Domain(r.dims, r.grid, r.voxel, r.dt, r.diffusion, r.decay, initial=r.initial,
-                                reduce=r.reduce)
-                if r.field is not None:
-                    d.upload_field(r.field)
-                if r.faces is not None:
-                    d.set_dirichlet_faces(*r.faces)
-                if r.bulk is not None:
-                    d.set_bulk(*r.bulk)
-                if r.agents is not None:
-                    d.set_agents(**r.agents)
+                d = domain_for(r)
                 r.admitted = self.now()
                 self.in_use += r.device_bytes()
                 self.active.append([r, d, 0])
@@ -167,6 +158,30 @@ class Server:
             self._batch.close()
             self._batch = None
         return self.done
+
+
+def domain_for(r):
+    """A bfvm.Domain holding request r's microenvironment (field, hooks, agents)."""
+    d = bfvm.Domain(r.dims, r.grid, r.voxel, r.dt, r.diffusion, r.decay, initial=r.initial, reduce=r.reduce)
+    if r.field is not None:
+        d.upload_field(r.field)
+    if r.faces is not None:
+        d.set_dirichlet_faces(*r.faces)
+    if r.bulk is not None:
+        d.set_bulk(*r.bulk)
+    if r.agents is not None:
+        d.set_agents(**r.agents)
+    return d
+
+
+def workload_of(r):
+    """Request r as a physicore_amd.workloads.Workload (its step count included)."""
+    from physicore_amd import workloads
+
+    return workloads.Workload(f"request {r.rid}", r.dims, tuple(r.grid), tuple(r.voxel), r.dt,
+                              np.asarray(r.diffusion), np.asarray(r.decay), r.steps,
+                              initial=None if r.initial is None else np.asarray(r.initial), field=r.field,
+                              faces=r.faces, bulk=r.bulk, agents=r.agents)
 
 
 def percentile(values, q):

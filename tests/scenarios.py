@@ -10,6 +10,7 @@ import numpy as np
 
 import oracle_path  # noqa: F401  (puts oracle/ on sys.path)
 import oracle as orc
+from physicore_amd import workloads as wl
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
@@ -119,64 +120,60 @@ def brute_force_internalized_1d(p, field_after, ag, internalized):
 
 # ------------------------------------------------------- benchmark workloads
 
-def _d_lambda(rng, S):
-    D = 10.0 ** rng.uniform(0.0, 5.0, S)
-    lam = rng.uniform(0.01, 1.0, S)
-    return D, lam
-
-
-def config2(side=200, S=4, n_cells=100_000, seed=43, with_cells=True, with_bulk=True):
-    """Single large domain: side^3 voxels at dx=20, S substrates with D log-uniform in
-    [1,1e5] and decay uniform in [0.01,1], field U[0,1], Dirichlet 1.0 on x-min, bulk
-    supply 0.5 (target 1, uptake 0) inside a central sphere of r=500um, uniform cells
-    with secretion/uptake U[0,10] (saturation 1, volume 2494)."""
-    rng = np.random.default_rng(seed)
-    dx = 20
-    box = side * dx
-    D, lam = _d_lambda(rng, S)
-    p = orc.Problem(3, (side, side, side), (dx, dx, dx), 0.01, D, lam)
-    field = rng.uniform(0.0, 1.0, p.shape)
-    dirs = orc.DirichletSpec(S, 3)
-    dirs.set_face(0, "min", np.ones(S), np.ones(S, dtype=np.uint8))
+def oracle_inputs(w, quirks=False):
+    """The oracle's inputs for a physicore_amd.workloads.Workload: (Problem, FP64
+    field, DirichletSpec or None, dense bulk (supply, uptake, target) or None,
+    AgentSpec or None).  The field is the workload's FP64 field (round it to FP32
+    to start from what the GPU holds)."""
+    S, dims = w.S, w.dims
+    p = orc.Problem(dims, w.grid, w.voxel, w.dt, w.diffusion, w.decay, openmp_quirks=quirks)
+    field = np.array(w.field_or_initial(), dtype=np.float64, copy=True)
+    dirs = None
+    if w.faces is not None or w.interior is not None:
+        dirs = orc.DirichletSpec(S, dims)
+        if w.faces is not None:
+            vals, conds = w.faces
+            for a in range(dims):
+                for k, side in enumerate(("min", "max")):
+                    if np.asarray(conds[2 * a + k]).any():
+                        dirs.set_face(a, side, vals[2 * a + k], conds[2 * a + k])
+        if w.interior is not None:
+            dirs.set_interior(*w.interior)
     bulk = None
-    if with_bulk:
-        c = (np.arange(side) + 0.5) * dx - box / 2.0
-        zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
-        inside = (xx ** 2 + yy ** 2 + zz ** 2) <= 500.0 ** 2
-        supply = np.zeros(p.shape)
-        supply[inside] = 0.5
-        target = np.zeros(p.shape)
-        target[inside] = 1.0
-        bulk = (supply, np.zeros(p.shape), target)
+    if w.bulk is not None:
+        vox, sup, upt, tgt = w.bulk
+        bulk = []
+        for src in (sup, upt, tgt):
+            dense = np.zeros(p.shape)
+            dense.reshape(-1, S)[np.asarray(vox)] = src
+            bulk.append(dense)
+        bulk = tuple(bulk)
     ag = None
-    if with_cells and n_cells > 0:
-        ag = orc.AgentSpec(n_cells, S, 3)
-        ag.positions[:] = rng.uniform(0.0, box, (n_cells, 3))
-        np.minimum(ag.positions, np.nextafter(box, 0), out=ag.positions)
-        ag.secretion[:] = rng.uniform(0.0, 10.0, (n_cells, S))
-        ag.uptake[:] = rng.uniform(0.0, 10.0, (n_cells, S))
-        ag.saturation[:] = 1.0
-        ag.volumes[:] = 2494.0
+    if w.agents is not None:
+        a = w.agents
+        ag = orc.AgentSpec(len(a["volumes"]), S, dims)
+        for k in ("positions", "secretion", "saturation", "uptake", "net_export", "fraction_released",
+                  "volumes", "is_active"):
+            setattr(ag, k, np.array(a[k], dtype=getattr(ag, k).dtype, copy=True).reshape(getattr(ag, k).shape))
+        if a.get("internalized") is not None:
+            ag.internalized = np.array(a["internalized"], dtype=np.float64, copy=True).reshape(ag.internalized.shape)
     return p, field, dirs, bulk, ag
 
 
+def config2(side=200, S=4, n_cells=100_000, seed=43, with_cells=True, with_bulk=True):
+    """workloads.c2 in the oracle's terms: side^3 voxels at dx=20, S substrates
+    with D log-uniform in [1,1e5] and decay uniform in [0.01,1], field U[0,1],
+    Dirichlet 1.0 on x-min, bulk supply 0.5 (target 1, uptake 0) inside a central
+    sphere of r=500um, uniform cells with secretion/uptake U[0,10] (saturation 1,
+    volume 2494)."""
+    return oracle_inputs(wl.c2(side, S, n_cells, seed, with_cells, with_bulk))
+
+
 def config1(seed=42):
-    """Plain CPU-runnable case: 50^3 at dx=20, S=1, D=1000, decay 0.1, dt=0.01, field 0,
-    Dirichlet 38 on all six faces, 1000 cells uniform (volume 2494, uptake 10,
-    saturation 38)."""
-    rng = np.random.default_rng(seed)
-    p = orc.Problem(3, (50, 50, 50), (20, 20, 20), 0.01, [1000.0], [0.1])
-    field = np.zeros(p.shape)
-    dirs = orc.DirichletSpec(1, 3)
-    for a in range(3):
-        dirs.set_face(a, "min", [38.0], [1])
-        dirs.set_face(a, "max", [38.0], [1])
-    ag = orc.AgentSpec(1000, 1, 3)
-    ag.positions[:] = rng.uniform(0.0, 1000.0, (1000, 3))
-    ag.uptake[:] = 10.0
-    ag.saturation[:] = 38.0
-    ag.volumes[:] = 2494.0
-    return p, field, dirs, None, ag
+    """workloads.c1 in the oracle's terms: 50^3 at dx=20, S=1, D=1000, decay 0.1,
+    dt=0.01, field 0, Dirichlet 38 on all six faces, 1000 cells uniform (volume
+    2494, uptake 10, saturation 38)."""
+    return oracle_inputs(wl.c1(seed))
 
 
 def mixed_domain(rng, sides, max_S=8, max_cells=50_000, dims=3, extras=False, dx=20):
@@ -188,7 +185,7 @@ def mixed_domain(rng, sides, max_S=8, max_cells=50_000, dims=3, extras=False, dx
     export, internalized tracking and clustered (shared-voxel) cells."""
     S = int(rng.integers(1, max_S + 1))
     n = [int(rng.choice(sides)) if a < dims else 1 for a in range(3)]
-    D, lam = _d_lambda(rng, S)
+    D, lam = wl._d_lambda(rng, S)
     p = orc.Problem(dims, tuple(n), (dx, dx, dx), 0.01, D, lam)
     field = rng.uniform(0.0, 1.0, p.shape)
     dirs = None
@@ -235,25 +232,8 @@ def mixed_domain(rng, sides, max_S=8, max_cells=50_000, dims=3, extras=False, dx
 
 
 def config4(seed=45, n_cells=2_000_000, nx=400, ny=400, nz=100):
-    """Dense agents: nx x ny x nz voxels at dx=10 (400x400x100: box 4000x4000x1000),
-    S=2 with D={1e5, 100}, decay {0.1, 0.01}, field U[0,1]; cells drawn from an
-    isotropic Gaussian (sigma 300um) at the box centre, redrawn until inside the
-    box; secretion/uptake U[0,10] (saturation 1, volume 2494), net export U[-1,1]."""
-    rng = np.random.default_rng(seed)
-    dx = 10
-    p = orc.Problem(3, (nx, ny, nz), (dx, dx, dx), 0.01, [1e5, 100.0], [0.1, 0.01])
-    field = rng.uniform(0.0, 1.0, p.shape)
-    box = np.array([nx * dx, ny * dx, nz * dx], dtype=np.float64)
-    pos = np.empty((0, 3))
-    while len(pos) < n_cells:
-        c = rng.normal(box / 2, 300.0, (n_cells, 3))
-        c = c[((c >= 0) & (c < box)).all(axis=1)]
-        pos = np.concatenate([pos, c])[:n_cells]
-    ag = orc.AgentSpec(n_cells, 2, 3)
-    ag.positions[:] = pos
-    ag.secretion[:] = rng.uniform(0.0, 10.0, (n_cells, 2))
-    ag.uptake[:] = rng.uniform(0.0, 10.0, (n_cells, 2))
-    ag.saturation[:] = 1.0
-    ag.volumes[:] = 2494.0
-    ag.net_export[:] = rng.uniform(-1.0, 1.0, (n_cells, 2))
-    return p, field, None, None, ag
+    """workloads.c4 in the oracle's terms: nx x ny x nz voxels at dx=10, S=2 with
+    D={1e5, 100}, decay {0.1, 0.01}, field U[0,1]; cells from an isotropic
+    Gaussian (sigma 300um) at the box centre, redrawn until inside the box;
+    secretion/uptake U[0,10] (saturation 1, volume 2494), net export U[-1,1]."""
+    return oracle_inputs(wl.c4(seed, n_cells, nx, ny, nz))
