@@ -73,8 +73,12 @@ def build_hip(jobs=None):
     return out
 
 
-HOST_SRCS = ["mesh.cpp", "agent_data.cpp", "microenvironment.cpp", "microenvironment_builder.cpp",
-             "solver_registry.cpp", "hip_solver.cpp", "vtk_serializer.cpp"]
+# the reference interface the backend plugs into (host/include, host/src) and
+# the backend itself (host/hip_solver: the sources that drop into the
+# reference tree as reactions-diffusion/biofvm/kernels/hip_solver)
+HOST_SRCS = ["src/mesh.cpp", "src/agent_data.cpp", "src/microenvironment.cpp", "src/solver_registry.cpp",
+             "src/vtk_serializer.cpp", "hip_solver/src/hip_solver.cpp"]
+HOST_INC = [os.path.join(HOST, "include"), os.path.join(HOST, "hip_solver", "include"), os.path.join(ROOT, "include")]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
@@ -84,9 +88,9 @@ def build_host():
     through libbfvm_hip.so's C ABI (no HIP headers)."""
     os.makedirs(LIB_DIR, exist_ok=True)
     out = os.path.join(LIB_DIR, "libbfvm_host.so")
-    srcs = [os.path.join(HOST, "src", f) for f in HOST_SRCS]
-    run([CXX, *HOST_FLAGS, "-shared", "-I", os.path.join(HOST, "include"), "-I", os.path.join(ROOT, "include"),
-         *srcs, "-L", LIB_DIR, "-lbfvm_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    srcs = [os.path.join(HOST, f) for f in HOST_SRCS]
+    inc = [x for d in HOST_INC for x in ("-I", d)]
+    run([CXX, *HOST_FLAGS, "-shared", *inc, *srcs, "-L", LIB_DIR, "-lbfvm_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 
@@ -99,8 +103,8 @@ def build_cpp_tests():
     bindir = os.path.join(ROOT, "tests", "cpp", "bin")
     os.makedirs(bindir, exist_ok=True)
     out = os.path.join(bindir, "test_hip_solver")
-    run([CXX, *HOST_FLAGS, "-I", os.path.join(HOST, "include"), "-I", os.path.join(ROOT, "include"),
-         src, "-L", LIB_DIR, "-lbfvm_host", "-lbfvm_hip", f"-Wl,-rpath,{LIB_DIR}",
+    inc = [x for d in HOST_INC for x in ("-I", d)]
+    run([CXX, *HOST_FLAGS, *inc, src, "-L", LIB_DIR, "-lbfvm_host", "-lbfvm_hip", f"-Wl,-rpath,{LIB_DIR}",
          "-Wl,-rpath,$ORIGIN/../../../physicore_amd/lib", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out

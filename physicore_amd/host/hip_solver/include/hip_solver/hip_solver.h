@@ -9,14 +9,18 @@
 // handed out through the non-const accessor is uploaded before the next solve.
 // Internalized substrates stay on the device until transfer_to_host, as with
 // the Thrust backend's data_manager (kernels/thrust_solver/src/data_manager.cpp:11-109).
+//
+// Includes only headers the reference tree has (common/, biofvm/ and
+// include/bfvm_hip.h of this library), so the same sources build in the
+// reference tree as a kernels/hip_solver backend.
 #pragma once
 
 #include <vector>
 
-#include <biofvm/agent.h>
+#include <biofvm/agent_container.h>
 #include <biofvm/microenvironment.h>
 #include <biofvm/solver.h>
-#include <common/agents.h>
+#include <common/generic_agent_solver.h>
 
 struct bfvm_ctx;
 

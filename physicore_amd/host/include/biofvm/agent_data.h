@@ -4,7 +4,7 @@
 
 #include <vector>
 
-#include <common/agents.h>
+#include <common/base_agent_data.h>
 #include <common/types.h>
 
 namespace physicore::biofvm {

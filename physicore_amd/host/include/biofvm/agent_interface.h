@@ -3,7 +3,7 @@
 
 #include <span>
 
-#include <common/agents.h>
+#include <common/base_agent_interface.h>
 #include <common/types.h>
 
 namespace physicore::biofvm {

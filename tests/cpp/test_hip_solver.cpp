@@ -1,5 +1,6 @@
 // The reference's solver tests, ported onto hip_solver through the public
-// biofvm interface (microenvironment, builder, solver, registry).
+// biofvm interface (microenvironment, solver, registry).  Microenvironments are
+// filled through their data contract by env_fixture.h.
 //
 // Sources (reference checkout): reactions-diffusion/biofvm/kernels/openmp_solver/tests/
 // test_diffusion_solver.cpp, test_dirichlet_solver.cpp, test_bulk_solver.cpp,
@@ -33,16 +34,27 @@
 
 #include <bfvm_hip.h>
 #include <biofvm/microenvironment.h>
-#include <biofvm/microenvironment_builder.h>
 #include <biofvm/solver_registry.h>
 #include <biofvm/vtk_serializer.h>
 #include <hip_solver/hip_solver.h>
 
+#include "env_fixture.h"
+#include <common/generic_agent_solver.h>
+
 using namespace physicore;
 using namespace physicore::biofvm;
 using physicore::biofvm::kernels::hip_solver::hip_solver;
+using fixture::env_spec;
+using fixture::make_env;
 
 // ------------------------------------------------------------- harness
+
+namespace {
+// the agent arrays of a microenvironment, reached as a solver reaches them
+struct agent_access : physicore::generic_agent_solver<agent>
+{};
+agent_data& agent_arrays_of(microenvironment& m) { return agent_access {}.retrieve_agent_data(*m.agents); }
+} // namespace
 
 namespace {
 struct test_case
@@ -379,13 +391,16 @@ TEST_GPU(DirichletSolverTest, Boundary3D) { boundary_case(3); }
 // test_dirichlet_solver.cpp:355-505, through the builder and the registry
 TEST_GPU(DirichletSolverTest, DynamicBoundaryModification)
 {
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 100.0, 0.01, 38.0);
-	builder.resize(3, { 0, 0, 0 }, { 100, 100, 100 }, { 10, 10, 10 });
-	builder.add_boundary_dirichlet_conditions(0, { 160.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { true, false, false },
-											  { false, false, false });
-	builder.select_solver("hip_solver");
-	auto env = builder.build();
+	env_spec sp;
+	sp.substrates.push_back({ "O2", "mmHg", 100.0, 0.01, 38.0 });
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 100, 100, 100 };
+	sp.voxel = { 10, 10, 10 };
+	sp.faces.push_back({ 0, { 160.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { true, false, false },
+											  { false, false, false } });
+	sp.solver = "hip_solver";
+	auto env = make_env(sp);
 	env->solver->initialize(*env);
 	for (int i = 0; i < 5; ++i)
 		env->run_single_timestep();
@@ -398,13 +413,16 @@ TEST_GPU(DirichletSolverTest, DynamicBoundaryModification)
 
 TEST_GPU(DirichletSolverTest, DynamicInteriorVoxelModification)
 {
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 100.0, 0.01, 38.0);
-	builder.add_density("Glucose", "mM", 50.0, 0.02, 5.0);
-	builder.resize(3, { 0, 0, 0 }, { 100, 100, 100 }, { 10, 10, 10 });
-	builder.add_dirichlet_node({ 5, 5, 5 }, { 160.0, 50.0 }, { true, true });
-	builder.select_solver("hip_solver");
-	auto env = builder.build();
+	env_spec sp;
+	sp.substrates.push_back({ "O2", "mmHg", 100.0, 0.01, 38.0 });
+	sp.substrates.push_back({ "Glucose", "mM", 50.0, 0.02, 5.0 });
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 100, 100, 100 };
+	sp.voxel = { 10, 10, 10 };
+	sp.interior.push_back({ { 5, 5, 5 }, { 160.0, 50.0 }, { true, true } });
+	sp.solver = "hip_solver";
+	auto env = make_env(sp);
 	env->solver->initialize(*env);
 	for (int i = 0; i < 5; ++i)
 		env->run_single_timestep();
@@ -417,13 +435,16 @@ TEST_GPU(DirichletSolverTest, DynamicInteriorVoxelModification)
 
 TEST_GPU(DirichletSolverTest, DisableDirichletConditionsDynamically)
 {
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 100.0, 0.01, 38.0);
-	builder.resize(3, { 0, 0, 0 }, { 100, 100, 100 }, { 10, 10, 10 });
-	builder.add_boundary_dirichlet_conditions(0, { 160.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { true, false, false },
-											  { false, false, false });
-	builder.select_solver("hip_solver");
-	auto env = builder.build();
+	env_spec sp;
+	sp.substrates.push_back({ "O2", "mmHg", 100.0, 0.01, 38.0 });
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 100, 100, 100 };
+	sp.voxel = { 10, 10, 10 };
+	sp.faces.push_back({ 0, { 160.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { true, false, false },
+											  { false, false, false } });
+	sp.solver = "hip_solver";
+	auto env = make_env(sp);
 	env->solver->initialize(*env);
 	for (int i = 0; i < 5; ++i)
 		env->run_single_timestep();
@@ -436,14 +457,17 @@ TEST_GPU(DirichletSolverTest, DisableDirichletConditionsDynamically)
 
 TEST_GPU(DirichletSolverTest, MultipleSubstrateModification)
 {
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 100.0, 0.01, 38.0);
-	builder.add_density("Glucose", "mM", 50.0, 0.02, 5.0);
-	builder.add_density("Lactate", "mM", 30.0, 0.015, 2.0);
-	builder.resize(3, { 0, 0, 0 }, { 100, 100, 100 }, { 10, 10, 10 });
-	builder.add_dirichlet_node({ 5, 5, 5 }, { 160.0, 50.0, 10.0 }, { true, true, true });
-	builder.select_solver("hip_solver");
-	auto env = builder.build();
+	env_spec sp;
+	sp.substrates.push_back({ "O2", "mmHg", 100.0, 0.01, 38.0 });
+	sp.substrates.push_back({ "Glucose", "mM", 50.0, 0.02, 5.0 });
+	sp.substrates.push_back({ "Lactate", "mM", 30.0, 0.015, 2.0 });
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 100, 100, 100 };
+	sp.voxel = { 10, 10, 10 };
+	sp.interior.push_back({ { 5, 5, 5 }, { 160.0, 50.0, 10.0 }, { true, true, true } });
+	sp.solver = "hip_solver";
+	auto env = make_env(sp);
 	env->solver->initialize(*env);
 	EXPECT_EQ(env->dirichlet_interior_voxels_count, (index_t)1);
 	for (int i = 0; i < 5; ++i)
@@ -604,7 +628,7 @@ void simple_case(index_t dims, bool internalized, bool recompute)
 // brute-force expectations of test_cell_solver.cpp:42-102 (1-D meshes)
 std::vector<real_t> expected_densities_1d(microenvironment& m)
 {
-	auto& data = dynamic_cast<agent_container&>(*m.agents).agent_data_ref();
+	auto& data = agent_arrays_of(m);
 	const index_t S = m.substrates_count, nx = m.mesh.grid_shape[0];
 	const real_t V = (real_t)m.mesh.voxel_volume(), dt = m.diffusion_timestep;
 	std::vector<real_t> out(nx * S);
@@ -627,7 +651,7 @@ std::vector<real_t> expected_densities_1d(microenvironment& m)
 
 void expected_internalized_1d(microenvironment& m, std::vector<real_t>& acc)
 {
-	auto& data = dynamic_cast<agent_container&>(*m.agents).agent_data_ref();
+	auto& data = agent_arrays_of(m);
 	const index_t S = m.substrates_count;
 	const real_t V = (real_t)m.mesh.voxel_volume(), dt = m.diffusion_timestep;
 	for (index_t i = 0; i < data.agents_count; i++)
@@ -645,7 +669,7 @@ void expected_internalized_1d(microenvironment& m, std::vector<real_t>& acc)
 
 void conflict_case(microenvironment& m, bool internalized, bool recompute)
 {
-	auto& data = dynamic_cast<agent_container&>(*m.agents).agent_data_ref();
+	auto& data = agent_arrays_of(m);
 	const index_t S = m.substrates_count, n = data.agents_count;
 	std::vector<real_t> exp_int(n * S, 0);
 	m.solver->initialize(m); // the expectations read the initial densities
@@ -827,25 +851,19 @@ TEST_HOST(SolverRegistryTest, RegistryAdder)
 	EXPECT_NE(solver_registry::instance().get("solver_x"), nullptr);
 }
 
-TEST_HOST(BuilderTest, UnknownSolverThrows)
-{
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 1.0, 0.01, 20.0);
-	builder.resize(3, { 0, 0, 0 }, { 10, 10, 10 }, { 1, 1, 1 });
-	builder.select_solver("no_such_solver");
-	EXPECT_THROW(builder.build(), std::runtime_error);
-}
-
 // test_dirichlet_modification.cpp: the host-side Dirichlet API
 TEST_HOST(DirichletModification, SetInteriorVoxel)
 {
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 1.0, 0.01, 20.0);
-	builder.add_density("Glucose", "mM", 0.5, 0.02, 5.0);
-	builder.resize(3, { 0, 0, 0 }, { 10, 10, 10 }, { 1, 1, 1 });
-	builder.add_dirichlet_node({ 5, 5, 5 }, { 100.0, 50.0 }, { true, true });
-	builder.select_solver("hip_solver");
-	auto env = builder.build();
+	env_spec sp;
+	sp.substrates.push_back({ "O2", "mmHg", 1.0, 0.01, 20.0 });
+	sp.substrates.push_back({ "Glucose", "mM", 0.5, 0.02, 5.0 });
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 10, 10, 10 };
+	sp.voxel = { 1, 1, 1 };
+	sp.interior.push_back({ { 5, 5, 5 }, { 100.0, 50.0 }, { true, true } });
+	sp.solver = "hip_solver";
+	auto env = make_env(sp);
 	EXPECT_EQ(env->dirichlet_interior_voxels_count, (index_t)1);
 	EXPECT_EQ(env->dirichlet_interior_values[0], 100.0);
 	EXPECT_EQ(env->dirichlet_interior_values[1], 50.0);
@@ -864,12 +882,15 @@ TEST_HOST(DirichletModification, SetInteriorVoxel)
 
 TEST_HOST(DirichletModification, SetBoundary)
 {
-	microenvironment_builder builder;
-	builder.add_density("O2", "mmHg", 1.0, 0.01, 20.0);
-	builder.add_density("Glucose", "mM", 0.5, 0.02, 5.0);
-	builder.resize(2, { 0, 0, 0 }, { 10, 10, 0 }, { 1, 1, 1 });
-	builder.select_solver("hip_solver");
-	auto env = builder.build();
+	env_spec sp;
+	sp.substrates.push_back({ "O2", "mmHg", 1.0, 0.01, 20.0 });
+	sp.substrates.push_back({ "Glucose", "mM", 0.5, 0.02, 5.0 });
+	sp.dims = 2;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 10, 10, 0 };
+	sp.voxel = { 1, 1, 1 };
+	sp.solver = "hip_solver";
+	auto env = make_env(sp);
 	EXPECT_TRUE(env->dirichlet_min_boundary_values[0] == nullptr);
 	env->update_dirichlet_boundary_min('x', 1, 5.0, true);
 	EXPECT_EQ(env->dirichlet_min_boundary_values[0][1], 5.0);
@@ -996,18 +1017,19 @@ std::string slurp(const std::filesystem::path& p)
 std::unique_ptr<microenvironment> vtk_env(index_t dims = 3, std::array<index_t, 3> grid = { 3, 3, 3 },
 										  std::array<index_t, 3> voxel = { 20, 20, 20 })
 {
-	microenvironment_builder builder;
-	builder.set_name("test_env");
-	builder.set_time_units("min");
-	builder.set_space_units("um");
-	builder.set_time_step(0.01);
-	builder.resize(dims, { 0, 0, 0 },
-				   { (sindex_t)(grid[0] * voxel[0]), (sindex_t)(grid[1] * voxel[1]), (sindex_t)(grid[2] * voxel[2]) },
-				   voxel);
-	builder.add_density("O2", "mmHg", 1.0, 0.01, 38.0);
-	builder.add_density("Glucose", "mM", 0.5, 0.02, 5.5);
-	builder.select_solver("hip_solver");
-	auto m = builder.build();
+	env_spec sp;
+	sp.name = "test_env";
+	sp.time_units = "min";
+	sp.space_units = "um";
+	sp.dt = 0.01;
+	sp.dims = dims;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { (sindex_t)(grid[0] * voxel[0]), (sindex_t)(grid[1] * voxel[1]), (sindex_t)(grid[2] * voxel[2]) };
+	sp.voxel = voxel;
+	sp.substrates.push_back({ "O2", "mmHg", 1.0, 0.01, 38.0 });
+	sp.substrates.push_back({ "Glucose", "mM", 0.5, 0.02, 5.5 });
+	sp.solver = "hip_solver";
+	auto m = make_env(sp);
 	m->solver->solve(*m, 1);
 	return m;
 }
@@ -1150,18 +1172,21 @@ TEST_GPU(VtkSerializerTest, SingleAndManySubstrates)
 {
 	const auto dir = fresh_dir("bfvm_vtk_many");
 	const std::vector<std::string> names = { "O2", "Glucose", "Lactate", "ATP", "CO2", "H2O" };
-	microenvironment_builder builder;
-	builder.set_name("many_substrates_env");
-	builder.resize(3, { 0, 0, 0 }, { 40, 40, 40 }, { 20, 20, 20 });
+	env_spec sp;
+	sp.name = "many_substrates_env";
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 40, 40, 40 };
+	sp.voxel = { 20, 20, 20 };
 	for (std::size_t i = 0; i < names.size(); ++i)
 	{
-		builder.add_density(names[i], "unit", 1.0, 0.01, static_cast<real_t>(10 + i));
-		builder.add_boundary_dirichlet_conditions(
+		sp.substrates.push_back({ names[i], "unit", 1.0, 0.01, static_cast<real_t>(10 + i) });
+		sp.faces.push_back({ 
 			i, { (real_t)(15 + i), (real_t)(14 + i), (real_t)(13 + i) },
-			{ (real_t)(20 + i), (real_t)(19 + i), (real_t)(18 + i) }, { true, true, true }, { true, true, true });
+			{ (real_t)(20 + i), (real_t)(19 + i), (real_t)(18 + i) }, { true, true, true }, { true, true, true } });
 	}
-	builder.select_solver("hip_solver");
-	auto m = builder.build();
+	sp.solver = "hip_solver";
+	auto m = make_env(sp);
 	m->solver->initialize(*m);
 	vtk_serializer ser(dir.string(), *m);
 	ser.serialize(*m, 0.0);
@@ -1181,14 +1206,17 @@ TEST_GPU(VtkSerializerTest, SingleAndManySubstrates)
 TEST_GPU(VtkSerializerTest, NonZeroBoundingBoxMins)
 {
 	const auto dir = fresh_dir("bfvm_vtk_offset");
-	microenvironment_builder builder;
-	builder.set_name("offset_env");
-	builder.resize(3, { 100, 200, 300 }, { 160, 260, 360 }, { 20, 20, 20 });
-	builder.add_density("O2", "mmHg", 1.0, 0.01, 35.0);
-	builder.add_boundary_dirichlet_conditions(0, { 38.0, 36.0, 34.0 }, { 42.0, 40.0, 38.0 }, { true, true, true },
-											  { true, true, true });
-	builder.select_solver("hip_solver");
-	auto m = builder.build();
+	env_spec sp;
+	sp.name = "offset_env";
+	sp.dims = 3;
+	sp.lo = { 100, 200, 300 };
+	sp.hi = { 160, 260, 360 };
+	sp.voxel = { 20, 20, 20 };
+	sp.substrates.push_back({ "O2", "mmHg", 1.0, 0.01, 35.0 });
+	sp.faces.push_back({ 0, { 38.0, 36.0, 34.0 }, { 42.0, 40.0, 38.0 }, { true, true, true },
+											  { true, true, true } });
+	sp.solver = "hip_solver";
+	auto m = make_env(sp);
 	m->solver->initialize(*m);
 	vtk_serializer ser(dir.string(), *m);
 	ser.serialize(*m, 0.0);
@@ -1204,15 +1232,18 @@ TEST_GPU(VtkSerializerTest, NonZeroBoundingBoxMins)
 TEST_GPU(VtkSerializerTest, BoundaryConditionsEffect)
 {
 	const auto dir = fresh_dir("bfvm_vtk_boundary");
-	microenvironment_builder builder;
-	builder.set_name("boundary_test_env");
-	builder.resize(3, { 0, 0, 0 }, { 60, 60, 60 }, { 20, 20, 20 });
-	builder.add_density("O2", "mmHg", 0, 0, 20.0);
-	builder.add_density("Glucose", "mM", 0.5, 0.02, 5.5);
-	builder.add_boundary_dirichlet_conditions(0, { 100.0, 20.0, 20.0 }, { 5.0, 20.0, 20.0 }, { true, false, false },
-											  { true, false, false });
-	builder.select_solver("hip_solver");
-	auto m = builder.build();
+	env_spec sp;
+	sp.name = "boundary_test_env";
+	sp.dims = 3;
+	sp.lo = { 0, 0, 0 };
+	sp.hi = { 60, 60, 60 };
+	sp.voxel = { 20, 20, 20 };
+	sp.substrates.push_back({ "O2", "mmHg", 0, 0, 20.0 });
+	sp.substrates.push_back({ "Glucose", "mM", 0.5, 0.02, 5.5 });
+	sp.faces.push_back({ 0, { 100.0, 20.0, 20.0 }, { 5.0, 20.0, 20.0 }, { true, false, false },
+											  { true, false, false } });
+	sp.solver = "hip_solver";
+	auto m = make_env(sp);
 	m->solver->solve(*m, 1);
 	vtk_serializer ser(dir.string(), *m);
 	ser.serialize(*m, 0.0);
@@ -1247,15 +1278,18 @@ TEST_GPU(VtkSerializerTest, SnapshotsWhileStepping)
 {
 	const auto dir = fresh_dir("bfvm_vtk_async");
 	auto make = [] {
-		microenvironment_builder builder;
-		builder.set_name("async_env");
-		builder.resize(3, { 0, 0, 0 }, { 800, 640, 480 }, { 20, 20, 20 });
-		builder.add_density("A", "u", 500.0, 0.1, 1.0);
-		builder.add_density("B", "u", 20.0, 0.5, 2.0);
-		builder.add_boundary_dirichlet_conditions(0, { 10.0, 0, 0 }, { 0, 0, 0 }, { true, false, false },
-												  { false, false, false });
-		builder.select_solver("hip_solver");
-		return builder.build();
+		env_spec sp;
+		sp.name = "async_env";
+		sp.dims = 3;
+		sp.lo = { 0, 0, 0 };
+		sp.hi = { 800, 640, 480 };
+		sp.voxel = { 20, 20, 20 };
+		sp.substrates.push_back({ "A", "u", 500.0, 0.1, 1.0 });
+		sp.substrates.push_back({ "B", "u", 20.0, 0.5, 2.0 });
+		sp.faces.push_back({ 0, { 10.0, 0, 0 }, { 0, 0, 0 }, { true, false, false },
+												  { false, false, false } });
+		sp.solver = "hip_solver";
+		return make_env(sp);
 	};
 	auto m = make();
 	auto twin = make();
