@@ -374,6 +374,22 @@ def test_bulk_dense_random_matches_oracle():
     assert rel_linf(d.field(), s.field).max() < 2e-6
 
 
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+def test_bulk_then_cells_closed_form(reduce):
+    """A voxel in the bulk list that holds a cell gets cells(bulk(D)), the stage
+    order of openmp_solver.cpp:27-37 (test_oracle pins the same closed form on
+    the oracle); alone the plain step runs bulk and cells as one launch."""
+    import test_oracle as to
+
+    p, f, bulk_arrays, ag, bulk, cell = to._bulk_and_cell_voxel_case()
+    d = gpu_domain(p, field=f, bulk=bulk_arrays, agents=ag, reduce=reduce)
+    d.step(1)
+    got = d.field(np.float64).reshape(-1)
+    want = cell(bulk(5.0))
+    assert abs(got[1] - want) <= 2e-7 * abs(want), (got[1], want, bulk(cell(5.0)))
+    assert got[0] == 5.0 and got[2] == 5.0
+
+
 # ---------------------------------------------------------------------- cells
 
 def _cells_problem(p):
