@@ -189,6 +189,19 @@ int bfvm_release_internalized(bfvm_ctx* ctx, int64_t agent_index);
  * reduced in FP64 on the device in a fixed order; out[S]. */
 int bfvm_total_mass(bfvm_ctx* ctx, double* out);
 
+/* bfvm_total_mass without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc).  The reduction's partial sums live in a static
+ * 128 KiB device array of the library (16384 doubles; every context the
+ * library accepts fits: it needs S * ceil(V / 65536) + S of them), one per GPU,
+ * held until the process ends.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (contexts on different streams or threads) race and must
+ * be serialized by the caller.  Pending releases of internalized substrates
+ * (bfvm_release_internalized) are applied by an upload, so they make this call
+ * fail with BFVM_ERR_STATE; a size beyond the array gives BFVM_ERR_UNSUPPORTED.
+ * Same result as bfvm_total_mass, bit for bit. */
+int bfvm_total_mass_static(bfvm_ctx* ctx, double* out);
+
 /* Per-owner reduced sums after the last recompute, for parity checks:
  * owners[n_owners] agent indices (ascending), num/den/fac[n_owners*S] FP64,
  * *is_conflict.  Pass NULL arrays to query *n_owners only. */

@@ -72,6 +72,7 @@ _SIGS = {
     "bfvm_download_internalized": ([C.c_void_p, f64p], C.c_int),
     "bfvm_release_internalized": ([C.c_void_p, C.c_int64], C.c_int),
     "bfvm_total_mass": ([C.c_void_p, f64p], C.c_int),
+    "bfvm_total_mass_static": ([C.c_void_p, f64p], C.c_int),
     "bfvm_cell_sums": ([C.c_void_p, i64p, i64p, f64p, f64p, f64p, C.POINTER(C.c_int32)], C.c_int),
     "bfvm_field_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "bfvm_stream": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
@@ -289,6 +290,12 @@ class Domain:
     def total_mass(self):
         out = np.zeros(self.S)
         self._check(lib().bfvm_total_mass(self._h, _p(out)))
+        return out
+
+    def total_mass_static(self):
+        """total_mass through bfvm_total_mass_static (no device allocation)."""
+        out = np.zeros(self.S)
+        self._check(lib().bfvm_total_mass_static(self._h, _p(out)))
         return out
 
     def cell_sums(self):

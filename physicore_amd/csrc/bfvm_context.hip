@@ -10,6 +10,7 @@
 #include <string>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -190,6 +191,7 @@ struct bfvm_ctx
 	std::vector<double> D, lam, init;
 
 	hipStream_t stream = nullptr;
+	int device = 0; // the HIP device current at bfvm_create
 	int num_cu = 256;
 
 	dvec<float> field;
@@ -1362,6 +1364,53 @@ int batch_run(bfvm_batch* b, int64_t iterations, int64_t graph_steps)
 
 } // namespace
 
+namespace {
+// Device memory of bfvm_total_mass_static: the mass reduction's per-block
+// partial sums and its result, in a file-scope array instead of buffers the
+// context allocates.  Every GPU holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for that
+// device (the array's name is not a device address on the host).
+constexpr int64_t kStaticScratchDoubles = 16384; // 128 KiB
+__device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr; // per device id; null until resolved
+
+// The scratch array's address on device `dev`, resolved once per device with
+// that device current (the caller's current device is restored).
+hipError_t static_scratch(int dev, double** out)
+{
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (dev < 0)
+		return hipErrorInvalidDevice;
+	if ((size_t)dev >= g_scratch_ptr.size())
+		g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+	if (!g_scratch_ptr[(size_t)dev])
+	{
+		int cur = 0;
+		hipError_t e = hipGetDevice(&cur);
+		if (e == hipSuccess && cur != dev)
+			e = hipSetDevice(dev);
+		double* p = nullptr;
+		if (e == hipSuccess)
+			e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+		if (cur != dev)
+		{
+			const hipError_t r = hipSetDevice(cur);
+			if (e == hipSuccess)
+				e = r;
+		}
+		if (e != hipSuccess)
+			return e;
+		if (!p)
+			return hipErrorInvalidSymbol;
+		g_scratch_ptr[(size_t)dev] = p;
+	}
+	*out = g_scratch_ptr[(size_t)dev];
+	return hipSuccess;
+}
+} // namespace
+
 extern "C" {
 
 const char* bfvm_last_error(const bfvm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
@@ -1416,6 +1465,7 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 
 	int dev = 0;
 	hipError_t e = hipGetDevice(&dev);
+	c->device = dev;
 	if (e == hipSuccess)
 		e = init_sweep_kernels();
 	if (e == hipSuccess)
@@ -1906,6 +1956,29 @@ int bfvm_total_mass(bfvm_ctx* c, double* out)
 	TRY_HIP(c, launch_mass(c->field.p, c->S, c->V, c->mass_partial.p, c->mass_out.p,
 						   (double)(c->voxel[0] * c->voxel[1] * c->voxel[2]), c->stream));
 	TRY_HIP(c, hipMemcpyAsync(out, c->mass_out.p, c->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	return BFVM_OK;
+}
+
+int bfvm_total_mass_static(bfvm_ctx* c, double* out)
+{
+	if (!c || !out)
+		return c ? c->fail(BFVM_ERR_INVALID, "null output") : BFVM_ERR_INVALID;
+	if (!c->rel_agent.empty())
+		return c->fail(BFVM_ERR_STATE, "releases are pending: applying them uploads their list (device memory); "
+									   "call bfvm_sync or bfvm_total_mass first");
+	const int64_t nb = mass_blocks(c->V);
+	const int64_t need = nb * c->S + c->S;
+	if (need > kStaticScratchDoubles)
+		return c->fail(BFVM_ERR_UNSUPPORTED, "the mass reduction needs " + std::to_string(need) +
+												 " doubles of scratch, more than the " +
+												 std::to_string(kStaticScratchDoubles) + " of the static array");
+	double* base = nullptr;
+	TRY_HIP(c, static_scratch(c->device, &base));
+	// every partial and the result are written before they are read: nothing to clear
+	TRY_HIP(c, launch_mass(c->field.p, c->S, c->V, base, base + nb * c->S,
+						   (double)(c->voxel[0] * c->voxel[1] * c->voxel[2]), c->stream));
+	TRY_HIP(c, hipMemcpyAsync(out, base + nb * c->S, c->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	return BFVM_OK;
 }
