@@ -117,7 +117,7 @@ def build_c3_domains(seed=44, count=64):
     return doms, units
 
 
-def bench_batched(steps, graph_steps=10, seed=44, cpu=True):
+def bench_batched(steps, graph_steps=10, seed=44, cpu=True, compare=True):
     """The 64-domain ensemble stepped as one batch (after autotuning) and, for
     comparison, the same domains stepped one after another on the plain path."""
     from physicore_amd import bfvm
@@ -133,6 +133,10 @@ def bench_batched(steps, graph_steps=10, seed=44, cpu=True):
     el = time.perf_counter() - t0
     info = b.info()
     batched = units * steps / el
+    if not compare:
+        b.close()
+        return {"workload": f"{len(doms)} domains, seed {seed}", "voxel_substrates": units, "steps": steps,
+                "value": batched, "ms_per_step": el * 1e3 / steps, "launches_per_step": info, "tuning": tuning}
     # plain: each domain alone (default tuning), one after the other
     for d in doms:
         d.set_tuning()
@@ -321,6 +325,8 @@ def main():
     ap.add_argument("--legs", default="plain,batched,served,reference",
                     help="comma-separated parts to run: plain, batched, served, reference")
     ap.add_argument("--reference-steps", type=int, default=100)
+    ap.add_argument("--batched-no-compare", action="store_true",
+                    help="batched leg without the one-domain-at-a-time comparison (for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MB device buffer 10 times (reference for HBM counters)")
     args = ap.parse_args()
@@ -353,7 +359,7 @@ def main():
     cpu = not args.no_cpu_baseline
     if "batched" in legs:
         print("bench.py: batched ensemble", file=sys.stderr, flush=True)
-        out["batched"] = bench_batched(args.batched_steps, cpu=cpu)
+        out["batched"] = bench_batched(args.batched_steps, cpu=cpu, compare=not args.batched_no_compare)
     if "served" in legs:
         print("bench.py: served stream", file=sys.stderr, flush=True)
         out["served"] = bench_served(args.served_requests, args.served_rate, cpu=cpu)

@@ -130,12 +130,27 @@ int bfvm_set_agents(bfvm_ctx* ctx, int64_t n, const double* positions, const dou
 					const uint8_t* is_active);
 
 /* solver::recompute_positional_data (openmp_solver.cpp:64): the next step
- * re-bins the agents and rebuilds the per-voxel sums. */
+ * re-bins the agents and rebuilds the per-voxel sums.  The recompute runs on
+ * the context stream before that step with no host wait: the owner count and
+ * the shared-voxel flag stay on the device, the step launches (and a captured
+ * graph) are kept.  An active agent outside the mesh is left out of the steps
+ * and reported (BFVM_ERR_OUT_OF_GRID) by the next call that waits for the
+ * stream (bfvm_sync, downloads, bfvm_total_mass, bfvm_cell_sums; for a batch,
+ * bfvm_batch_sync); the positional data then stays marked for recompute. */
 int bfvm_recompute_positional_data(bfvm_ctx* ctx);
+
+/* Device address of the agent positions (n*dims FP64, the layout of
+ * bfvm_set_agents) for callers that move agents on the GPU: write new
+ * positions there, ordered on the context stream (bfvm_stream), then call
+ * bfvm_recompute_positional_data.  Nothing is copied from the host.  No
+ * counterpart in the reference, whose agent arrays live in host memory
+ * (data_manager::transfer_to_device copies them, thrust_solver/src/data_manager.cpp:61-93). */
+int bfvm_agent_positions_device_ptr(bfvm_ctx* ctx, double** ptr);
 
 /* openmp_solver::solve (openmp_solver.cpp:23-40): `iterations` steps of
  * diffusion -> Dirichlet -> bulk -> cells, enqueued on the context stream.
- * Returns once enqueued; errors of a recompute are reported synchronously. */
+ * Returns once enqueued (an agent outside the mesh found by a recompute is
+ * reported by the next call that waits for the stream). */
 int bfvm_step(bfvm_ctx* ctx, int64_t iterations);
 
 /* Same, replayed from a HIP graph that captures `graph_steps` steps (the graph
@@ -226,9 +241,16 @@ int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
  * that axis (no counterpart in the reference). */
 int bfvm_time_sweep(bfvm_ctx* ctx, int32_t axis, int64_t launches, double* out_us);
 
-/* Kernel of the x sweep.  A kernel that does not take the shape falls back to
- * the strided lines; rows longer than 256 voxels run on the scan kernel where
- * it fits (up to 512 voxels, S*ceil(nx/64) <= 32), else inside LDS tiles. */
+/* Kernel of the x sweep.  BFVM_X_AUTO picks per shape (measured on MI355X):
+ * the chunked kernels for S = 4, 8 on rows of 96..200 voxels (and S = 4 up to
+ * 32), for S = 3 and (from 96 voxels) S = 5, 6, and for S = 2 on even rows up
+ * to 200 voxels; for S = 2 rows of 201..400 voxels the two-lane chunked
+ * kernel; the scan kernel for S = 1 and other rows past 200 voxels where it
+ * fits (up to 512 voxels, S*ceil(nx/64) <= 32); the strided lines otherwise.
+ * When the chunked choice does not take the shape (fewer rows than a wave
+ * holds, a 1-D domain with faces), the scan kernel runs where it fits.  An
+ * explicitly chosen kernel that does not take the shape falls back to the
+ * strided lines (rows up to 256 voxels) or LDS tiles. */
 typedef enum bfvm_x_sweep
 {
 	BFVM_X_AUTO = 0,	  /* the library's default for the shape */

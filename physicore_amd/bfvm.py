@@ -58,6 +58,7 @@ _SIGS = {
     "bfvm_set_bulk": ([C.c_void_p, C.c_int64, i64p, f64p, f64p, f64p], C.c_int),
     "bfvm_set_agents": ([C.c_void_p, C.c_int64] + [f64p] * 8 + [u8p], C.c_int),
     "bfvm_recompute_positional_data": ([C.c_void_p], C.c_int),
+    "bfvm_agent_positions_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "bfvm_step": ([C.c_void_p, C.c_int64], C.c_int),
     "bfvm_step_graph": ([C.c_void_p, C.c_int64, C.c_int64], C.c_int),
     "bfvm_sync": ([C.c_void_p], C.c_int),
@@ -233,6 +234,12 @@ class Domain:
 
     def recompute_positional_data(self):
         self._check(lib().bfvm_recompute_positional_data(self._h))
+
+    def agent_positions_device_ptr(self):
+        """Device address of the agent positions (n*dims FP64)."""
+        p = C.c_void_p()
+        self._check(lib().bfvm_agent_positions_device_ptr(self._h, C.byref(p)))
+        return p.value
 
     # ----------------------------------------------------------------- step
     def step(self, iterations=1):

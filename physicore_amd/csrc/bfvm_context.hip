@@ -233,6 +233,13 @@ struct bfvm_ctx
 	dvec<uint32_t> keys, vals, tkeys, tvals;
 	dvec<int32_t> hist, scan_tmp, heads, head_scan;
 	bool recompute_pending = false;
+	// a recompute was enqueued: the next call that waits for the stream reads
+	// its out-of-grid flag
+	bool grid_check_pending = false;
+	// positions were written on the device since the host copy was taken
+	bool h_positions_stale = false;
+	// owner count and shared-voxel flag of the last recompute, read back only
+	// for bfvm_cell_sums (the step kernels read them on the device)
 	int64_t n_owners = 0;
 	bool conflict = false;
 	// releases not yet applied to the field: (agent, voxel) in call order
@@ -521,7 +528,10 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepXGT;
 				c->nmax[axis] = xgt_bucket(n, c->S);
 			}
-			else if (axis == 0 && xs == BFVM_X_SCAN && vpl && !(c->dims == 1 && c->face_mask))
+			// the scan kernel: asked for, or the default's chunked choice did not
+			// take the shape (too few rows for a wave, or 1-D with faces)
+			else if (axis == 0 && vpl && !(c->dims == 1 && c->face_mask) &&
+					 (xs == BFVM_X_SCAN || (c->tuning.x_sweep == BFVM_X_AUTO && xs == BFVM_X_CHUNKED)))
 			{
 				c->variant[axis] = kSweepScan;
 				c->nmax[axis] = vpl;
@@ -624,19 +634,23 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev, const DomainDev* dd = nullptr, uin
 	TRY_HIP(c, mark(4));
 	// bulk and cell apply as one launch when both run (voxels in both lists
 	// are applied in order by the bulk side); the stage timer then counts the
-	// cell apply with bulk
-	const bool both = c->bulk_count && c->n_bulk_tiles > 0 && c->n_owners > 0 && c->n_cell_tiles > 0 && c->overlap;
+	// cell apply with bulk.  The cell tiles cover every agent slot; the kernels
+	// stop at the owner count of the last recompute, which stays on the device
+	// (as does the shared-voxel flag the internalized update checks), so the
+	// launches do not change when the agents move
+	const bool cells = c->n_agents > 0 && c->n_cell_tiles > 0;
+	const bool both = c->bulk_count && c->n_bulk_tiles > 0 && cells && c->overlap;
 	if (both)
 		TRY_HIP(c, launch_stages(dd, c->bulk_tiles.p, c->n_bulk_tiles, c->cell_tiles.p, c->n_cell_tiles,
 								 c->stream));
 	else if (c->bulk_count)
 		TRY_HIP(c, launch_bulk(dd, c->bulk_tiles.p, c->n_bulk_tiles, c->stream));
 	TRY_HIP(c, mark(5));
-	if (c->n_owners > 0)
+	if (cells)
 	{
 		if (!both)
 			TRY_HIP(c, launch_cells_apply(dd, c->cell_tiles.p, c->n_cell_tiles, c->stream));
-		if (c->internalized && c->conflict)
+		if (c->internalized)
 			TRY_HIP(c, launch_cells_internalized(dd, c->int_tiles.p, c->n_int_tiles, c->stream));
 	}
 	if (tick)
@@ -650,9 +664,9 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev, const DomainDev* dd = nullptr, uin
 int rebuild_overlap(bfvm_ctx* c)
 {
 	c->overlap = false;
-	if (c->bulk_count == 0 || c->n_owners == 0 || c->recompute_pending || !c->bulk_map.p)
+	if (c->bulk_count == 0 || c->n_agents == 0 || c->recompute_pending || !c->bulk_map.p)
 		return BFVM_OK;
-	TRY_HIP(c, launch_bulk_overlap(c->ddom.p, c->bulk_map.p, c->bulk_cell.p, c->bulk_count, c->n_owners, c->stream));
+	TRY_HIP(c, launch_bulk_overlap(c->ddom.p, c->bulk_map.p, c->bulk_cell.p, c->bulk_count, c->n_agents, c->stream));
 	c->overlap = true;
 	return BFVM_OK;
 }
@@ -670,38 +684,30 @@ int bits_for(uint64_t v)
 int recompute(bfvm_ctx* c)
 {
 	c->recompute_pending = false;
-	c->n_owners = 0;
-	c->conflict = false;
 	c->overlap = false;
 	const int64_t n = c->n_agents;
 	if (n == 0)
 		return BFVM_OK;
+	// everything below is enqueued on the stream: the owner count and the
+	// flags stay on the device (the sort buffers were sized by bfvm_set_agents)
 	hipStream_t s = c->stream;
 	TRY_HIP(c, hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int32_t), s));
 	if (c->reduce_mode == BFVM_REDUCE_ATOMIC)
 	{
-		if (!c->ballot.p)
-		{
-			TRY_HIP(c, c->ballot.alloc((size_t)c->V));
-			int rc = sync_domain(c);
-			if (rc)
-				return rc;
-		}
 		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, true, s));
 		TRY_HIP(c, launch_cells_ballot(c->ddom.p, n, s));
 		TRY_HIP(c, launch_cells_collect_owners(c->ddom.p, n, s));
+		// the atomics collected the owners in no particular order: sort the
+		// list by voxel so that the per-step apply walks the field in order
+		// (the list order does not change any result).  The list has a slot
+		// per agent; slots past the owner count sort last
+		TRY_HIP(c, launch_cells_owner_keys(c->ddom.p, c->keys.p, c->vals.p, n, s));
+		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, n,
+									bits_for((uint64_t)c->V), s));
+		TRY_HIP(c, hipMemcpyAsync(c->owners.p, c->vals.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
 	}
 	else
 	{
-		TRY_HIP(c, c->keys.alloc(n));
-		TRY_HIP(c, c->vals.alloc(n));
-		TRY_HIP(c, c->tkeys.alloc(n));
-		TRY_HIP(c, c->tvals.alloc(n));
-		TRY_HIP(c, c->heads.alloc(n));
-		TRY_HIP(c, c->head_scan.alloc(n));
-		TRY_HIP(c, c->hist.alloc(radix_hist_entries(n)));
-		const size_t st = std::max(scan_tmp_entries(n), scan_tmp_entries((int64_t)radix_hist_entries(n)));
-		TRY_HIP(c, c->scan_tmp.alloc(st));
 		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, false, s));
 		TRY_HIP(c, launch_cells_make_keys(c->ddom.p, c->keys.p, c->vals.p, n, s));
 		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, n,
@@ -711,41 +717,28 @@ int recompute(bfvm_ctx* c)
 		TRY_HIP(c, exclusive_scan(c->head_scan.p, c->scan_tmp.p, n, s));
 		TRY_HIP(c, launch_cells_segment_reduce(c->ddom.p, c->keys.p, c->vals.p, c->heads.p, c->head_scan.p, n, s));
 	}
+	TRY_HIP(c, launch_cells_pack(c->ddom.p, n, s));
+	TRY_HIP(c, launch_cells_agent_slots(c->ddom.p, n, s));
+	c->grid_check_pending = true;
+	return rebuild_overlap(c);
+}
+
+// After the stream has drained: reports an active agent outside the mesh
+// found by the last recompute (the agent took no part in the steps since; the
+// positional data stays marked for another recompute).
+int check_recompute(bfvm_ctx* c)
+{
+	if (!c->grid_check_pending)
+		return BFVM_OK;
+	c->grid_check_pending = false;
 	int32_t f[4] = { 0, 0, 0, 0 };
-	TRY_HIP(c, hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, s));
-	TRY_HIP(c, hipStreamSynchronize(s));
+	TRY_HIP(c, hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
+	c->n_owners = f[2];
+	c->conflict = f[0] != 0;
 	if (f[1])
 	{
 		c->recompute_pending = true;
 		return c->fail(BFVM_ERR_OUT_OF_GRID, "an active agent lies outside the mesh");
-	}
-	c->conflict = f[0] != 0;
-	c->n_owners = f[2];
-	if (c->reduce_mode == BFVM_REDUCE_ATOMIC && c->n_owners > 1)
-	{
-		// the atomics collected the owners in no particular order: sort the
-		// list by voxel so that the per-step apply walks the field in order
-		// (the list order does not change any result)
-		const int64_t m = c->n_owners;
-		TRY_HIP(c, c->keys.alloc(n));
-		TRY_HIP(c, c->vals.alloc(n));
-		TRY_HIP(c, c->tkeys.alloc(n));
-		TRY_HIP(c, c->tvals.alloc(n));
-		TRY_HIP(c, c->hist.alloc(radix_hist_entries(n)));
-		TRY_HIP(c, c->scan_tmp.alloc(std::max(scan_tmp_entries(n), scan_tmp_entries((int64_t)radix_hist_entries(n)))));
-		TRY_HIP(c, launch_cells_owner_keys(c->ddom.p, c->keys.p, c->vals.p, m, s));
-		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, m,
-									bits_for((uint64_t)c->V), s));
-		TRY_HIP(c, hipMemcpyAsync(c->owners.p, c->vals.p, m * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-	}
-	if (c->n_owners > 0)
-	{
-		TRY_HIP(c, launch_cells_pack(c->ddom.p, c->n_owners, s));
-		TRY_HIP(c, launch_cells_agent_slots(c->ddom.p, n, s));
-		auto t = sparse_tiles(0, c->n_owners, c->S);
-		c->n_cell_tiles = (int)t.size();
-		TRY_HIP(c, c->cell_tiles.upload(t.data(), t.size(), s)); // synchronizes s
-		return rebuild_overlap(c);
 	}
 	return BFVM_OK;
 }
@@ -963,9 +956,7 @@ std::vector<uint64_t> batch_signature(const bfvm_batch* b)
 	for (const bfvm_ctx* c : b->ctx)
 	{
 		sig.push_back(c->version);
-		sig.push_back((uint64_t)c->n_owners);
 		sig.push_back((uint64_t)c->n_agents);
-		sig.push_back(c->conflict ? 1 : 0);
 		sig.push_back((uint64_t)c->dir_count);
 		sig.push_back((uint64_t)c->bulk_count);
 	}
@@ -1103,8 +1094,8 @@ int batch_prepare(bfvm_batch* b)
 	constexpr int K = bfvm_batch::kBatchLanes;
 	auto stage_work = [&](int i) -> int64_t {
 		const bfvm_ctx* c = b->ctx[i];
-		const int64_t entries = c->dir_count + c->bulk_count + c->n_owners +
-								(c->internalized && c->conflict ? c->n_agents : 0);
+		// owners are at most the agents (their count stays on the device)
+		const int64_t entries = c->dir_count + c->bulk_count + c->n_agents + (c->internalized ? c->n_agents : 0);
 		return 10 * entries * c->S;
 	};
 	std::vector<int64_t> work(b->sweeps.size());
@@ -1192,10 +1183,10 @@ int batch_prepare(bfvm_batch* b)
 			}
 			int rc = make_tiles(axis, lane, [](const bfvm_ctx* c) { return c->bulk_count; }, st.bulk);
 			if (!rc)
-				rc = make_tiles(axis, lane, [](const bfvm_ctx* c) { return c->n_owners; }, st.cells);
+				rc = make_tiles(axis, lane, [](const bfvm_ctx* c) { return c->n_agents; }, st.cells);
 			if (!rc)
 				rc = make_tiles(axis, lane,
-								[](const bfvm_ctx* c) { return c->internalized && c->conflict ? c->n_agents : 0; },
+								[](const bfvm_ctx* c) { return c->internalized ? c->n_agents : 0; },
 								st.internalized);
 			if (rc)
 				return rc;
@@ -1705,10 +1696,27 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->pvox.alloc(n));
 	TRY_HIP(c, c->cell_in_bulk.alloc(n));
 	TRY_HIP(c, c->slot_of.alloc(n));
+	// the recompute's sort buffers (the owner list has a slot per agent)
+	TRY_HIP(c, c->keys.alloc(n));
+	TRY_HIP(c, c->vals.alloc(n));
+	TRY_HIP(c, c->tkeys.alloc(n));
+	TRY_HIP(c, c->tvals.alloc(n));
+	TRY_HIP(c, c->heads.alloc(n));
+	TRY_HIP(c, c->head_scan.alloc(n));
+	TRY_HIP(c, c->hist.alloc(radix_hist_entries(n)));
+	TRY_HIP(c, c->scan_tmp.alloc(std::max(scan_tmp_entries(n), scan_tmp_entries((int64_t)radix_hist_entries(n)))));
+	if (c->reduce_mode == BFVM_REDUCE_ATOMIC && n > 0)
+		TRY_HIP(c, c->ballot.alloc((size_t)c->V));
+	// per-agent tiles (internalized update) and tiles over the owner list's
+	// slots (cell apply; the kernels stop at the owner count)
 	auto t = sparse_tiles(0, n, c->S);
 	c->n_int_tiles = (int)t.size();
 	TRY_HIP(c, c->int_tiles.upload(t.data(), t.size(), c->stream));
+	c->n_cell_tiles = (int)t.size();
+	TRY_HIP(c, c->cell_tiles.upload(t.data(), t.size(), c->stream));
 	c->recompute_pending = true;
+	c->grid_check_pending = false;
+	c->h_positions_stale = false;
 	c->n_owners = 0;
 	c->overlap = false;
 	return sync_domain(c);
@@ -1718,8 +1726,21 @@ int bfvm_recompute_positional_data(bfvm_ctx* c)
 {
 	if (!c)
 		return BFVM_ERR_INVALID;
+	// the recompute runs on the stream before the next step; the step launches
+	// (and a captured graph) stay as they are
 	c->recompute_pending = true;
-	c->version++;
+	return BFVM_OK;
+}
+
+int bfvm_agent_positions_device_ptr(bfvm_ctx* c, double** ptr)
+{
+	if (!c || !ptr)
+		return BFVM_ERR_INVALID;
+	if (c->n_agents == 0)
+		return c->fail(BFVM_ERR_STATE, "no agents set");
+	FLUSH_RELEASES(c); // releases resolve voxels from the positions they were asked at
+	*ptr = c->positions.p;
+	c->h_positions_stale = true;
 	return BFVM_OK;
 }
 
@@ -1802,7 +1823,7 @@ int bfvm_sync(bfvm_ctx* c)
 		return BFVM_ERR_INVALID;
 	FLUSH_RELEASES(c);
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	return BFVM_OK;
+	return check_recompute(c);
 }
 
 int bfvm_download_field_f32(bfvm_ctx* c, float* host)
@@ -1812,7 +1833,7 @@ int bfvm_download_field_f32(bfvm_ctx* c, float* host)
 	FLUSH_RELEASES(c);
 	TRY_HIP(c, hipMemcpyAsync(host, c->field.p, c->field.n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	return BFVM_OK;
+	return check_recompute(c);
 }
 
 int bfvm_snapshot_begin(bfvm_ctx* c, float* host, int64_t* ticket)
@@ -1926,7 +1947,7 @@ int bfvm_download_internalized(bfvm_ctx* c, double* host)
 	TRY_HIP(c, hipMemcpyAsync(host, c->internalized_d.p, (size_t)c->n_agents * c->S * sizeof(double),
 							  hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	return BFVM_OK;
+	return check_recompute(c);
 }
 
 int bfvm_release_internalized(bfvm_ctx* c, int64_t agent)
@@ -1937,6 +1958,14 @@ int bfvm_release_internalized(bfvm_ctx* c, int64_t agent)
 		return BFVM_OK; // cell_solver.cpp:353-354
 	if (agent < 0 || agent >= c->n_agents)
 		return c->fail(BFVM_ERR_INVALID, "agent index out of range");
+	if (c->h_positions_stale)
+	{
+		// positions were written on the device (bfvm_agent_positions_device_ptr)
+		TRY_HIP(c, hipStreamSynchronize(c->stream));
+		TRY_HIP(c, hipMemcpy(c->h_positions.data(), c->positions.p, c->h_positions.size() * sizeof(double),
+							 hipMemcpyDeviceToHost));
+		c->h_positions_stale = false;
+	}
 	int64_t v;
 	if (!voxel_of(c, c->h_positions.data() + agent * c->dims, &v))
 		return c->fail(BFVM_ERR_OUT_OF_GRID, "agent outside the mesh");
@@ -1957,7 +1986,7 @@ int bfvm_total_mass(bfvm_ctx* c, double* out)
 						   (double)(c->voxel[0] * c->voxel[1] * c->voxel[2]), c->stream));
 	TRY_HIP(c, hipMemcpyAsync(out, c->mass_out.p, c->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	return BFVM_OK;
+	return check_recompute(c);
 }
 
 int bfvm_total_mass_static(bfvm_ctx* c, double* out)
@@ -1980,7 +2009,7 @@ int bfvm_total_mass_static(bfvm_ctx* c, double* out)
 						   (double)(c->voxel[0] * c->voxel[1] * c->voxel[2]), c->stream));
 	TRY_HIP(c, hipMemcpyAsync(out, base + nb * c->S, c->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	return BFVM_OK;
+	return check_recompute(c);
 }
 
 int bfvm_cell_sums(bfvm_ctx* c, int64_t* n_owners, int64_t* owners, double* num, double* den, double* fac,
@@ -1994,6 +2023,10 @@ int bfvm_cell_sums(bfvm_ctx* c, int64_t* n_owners, int64_t* owners, double* num,
 		if (rc)
 			return rc;
 	}
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	c->grid_check_pending = c->n_agents > 0; // read the count and flags of the last recompute
+	if (int rc = check_recompute(c))
+		return rc;
 	*n_owners = c->n_owners;
 	if (is_conflict)
 		*is_conflict = c->conflict ? 1 : 0;
@@ -2234,6 +2267,9 @@ int bfvm_batch_sync(bfvm_batch* b)
 		return BFVM_ERR_INVALID;
 	if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess)
 		return b->hip_fail(e, "hipStreamSynchronize");
+	for (size_t i = 0; i < b->ctx.size(); i++)
+		if (int rc = check_recompute(b->ctx[i]))
+			return b->fail(rc, "domain " + std::to_string(i) + ": " + b->ctx[i]->err);
 	return BFVM_OK;
 }
 

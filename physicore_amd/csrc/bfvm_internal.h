@@ -332,8 +332,9 @@ hipError_t launch_stages(const DomainDev* dom, const TileRef* bulk_tiles, int n_
 hipError_t launch_bulk_map(const int64_t* bulk_vox, int64_t count, int32_t* map, int64_t V, hipStream_t stream);
 // The domain's overlap maps (bulk_cell: the domain's, on the device) from the
 // voxel map and its apply list.
+// n: the agent count (the owner list's capacity; the owner count is read on the device)
 hipError_t launch_bulk_overlap(const DomainDev* dom, const int32_t* map, int32_t* bulk_cell, int64_t bulk_count,
-							   int64_t n_owners, hipStream_t stream);
+							   int64_t n, hipStream_t stream);
 hipError_t launch_fill_field(float* field, const float* init, int S, int64_t nvox, hipStream_t stream);
 hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream);
 hipError_t launch_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t stream);
@@ -345,13 +346,14 @@ hipError_t launch_release_flush(const DomainDev* dom, const int64_t* groups, int
 hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream);
 hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stream);
 hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream);
-// packs the apply list from owners[0..n_owners) (both modes)
-hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream);
+// packs the apply list from owners[0..flags[2]) (both modes); n: the agent count
+hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, hipStream_t stream);
 // After the pack: owner_of[i] becomes the slot of agent i's owner (-1 stays).
 hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream);
 // atomic mode: keys/vals = (voxel, owner) of the collected owner list, to sort
-// it into voxel order (radix_sort_pairs)
-hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n_owners,
+// it into voxel order (radix_sort_pairs); the n slots past the owner count
+// (read on the device) take key V and sort last
+hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
 								   hipStream_t stream);
 // sorted mode: keys/vals are scratch of n entries (sorted in place by radix_sort_pairs)
 hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,

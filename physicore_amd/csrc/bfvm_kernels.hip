@@ -276,24 +276,32 @@ __global__ void cells_segment_reduce_kernel(const DomainDev* __restrict__ dp, co
 // sums, so the per-step kernels read them coalesced.
 // Atomic mode: (voxel, owner agent) of each collected owner, to put the owner
 // list in voxel order (the collection order is the order of the atomics).
+// Slots of the list past the owner count (the list is sized for every agent,
+// the count stays on the device) take the key V, so they sort last.
 __global__ void cells_owner_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals,
-										int64_t n_owners)
+										int64_t n)
 {
 	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n_owners)
+	if (k >= n)
 		return;
 	const DomainDev& d = *dp;
+	if (k >= d.flags[2])
+	{
+		keys[k] = (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz;
+		vals[k] = 0;
+		return;
+	}
 	const int o = d.owners[k];
 	keys[k] = (uint32_t)d.cell_voxel[o];
 	vals[k] = (uint32_t)o;
 }
 
-__global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n_owners)
+__global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n)
 {
 	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n_owners)
-		return;
 	const DomainDev& d = *dp;
+	if (k >= n || k >= d.flags[2])
+		return;
 	const int o = d.owners[k];
 	const int S = d.S;
 	d.pvox[k] = d.cell_voxel[o];
@@ -343,6 +351,13 @@ __device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, d
 	return sr_round(d1, mix32((uint32_t)e ^ seed), nearest);
 }
 
+// Entries of tile tr that are owners: the tiles cover every agent slot of the
+// list, and the owner count of the last recompute stays on the device.
+__device__ __forceinline__ int owners_in_tile(const DomainDev& d, const TileRef& tr)
+{
+	return max(0, min(tr.b, d.flags[2] - tr.a));
+}
+
 __global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
 	const TileRef tr = tiles[blockIdx.x];
@@ -351,7 +366,8 @@ __global__ void cells_apply_kernel(const DomainDev* __restrict__ doms, const Til
 	const bool fused = d.compute_internalized && !d.flags[0];
 	const uint32_t seed = stage_seed(d, kRoundCells);
 	const bool nearest = d.round_nearest != 0;
-	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+	const int cnt = owners_in_tile(d, tr);
+	for (int t = threadIdx.x; t < cnt * S; t += blockDim.x)
 	{
 		const int k = tr.a + t / S, s = t % S;
 		float* D = d.field + (int64_t)d.pvox[k] * S + s;
@@ -392,7 +408,8 @@ __global__ void stages_kernel(const DomainDev* __restrict__ dom, const TileRef* 
 	else
 	{
 		const TileRef tr = ctiles[blockIdx.x - nb];
-		for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+		const int cnt = owners_in_tile(d, tr);
+		for (int t = threadIdx.x; t < cnt * S; t += blockDim.x)
 		{
 			const int k = tr.a + t / S, s = t % S;
 			if (d.cell_in_bulk[k])
@@ -418,11 +435,11 @@ __global__ void bulk_cell_clear_kernel(int32_t* bulk_cell, int64_t count)
 }
 
 __global__ void bulk_overlap_kernel(const DomainDev* __restrict__ dom, const int32_t* __restrict__ map,
-									int64_t n_owners)
+									int64_t n)
 {
 	const DomainDev& d = *dom;
 	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n_owners)
+	if (k >= n || k >= d.flags[2])
 		return;
 	const int32_t b = map[d.pvox[k]];
 	d.cell_in_bulk[k] = b >= 0;
@@ -436,6 +453,8 @@ __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, co
 {
 	const TileRef tr = tiles[blockIdx.x];
 	const DomainDev& d = doms[tr.dom];
+	if (!d.flags[0]) // no shared voxel: the apply updated the owners' amounts
+		return;
 	const int S = d.S;
 	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 	{
@@ -765,12 +784,12 @@ hipError_t launch_bulk_map(const int64_t* bulk_vox, int64_t count, int32_t* map,
 }
 
 hipError_t launch_bulk_overlap(const DomainDev* dom, const int32_t* map, int32_t* bulk_cell, int64_t bulk_count,
-							   int64_t n_owners, hipStream_t stream)
+							   int64_t n, hipStream_t stream)
 {
 	if (hipError_t e = launch1d(bulk_cell_clear_kernel, bulk_count, 256, stream, bulk_cell, bulk_count);
 		e != hipSuccess)
 		return e;
-	return launch1d(bulk_overlap_kernel, n_owners, 256, stream, dom, map, n_owners);
+	return launch1d(bulk_overlap_kernel, n, 256, stream, dom, map, n);
 }
 
 hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles, int ntiles, hipStream_t stream)
@@ -781,15 +800,15 @@ hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles
 	return hipGetLastError();
 }
 
-hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n_owners,
+hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
 								   hipStream_t stream)
 {
-	return launch1d(cells_owner_keys_kernel, n_owners, 256, stream, dom, keys, vals, n_owners);
+	return launch1d(cells_owner_keys_kernel, n, 256, stream, dom, keys, vals, n);
 }
 
-hipError_t launch_cells_pack(const DomainDev* dom, int64_t n_owners, hipStream_t stream)
+hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, hipStream_t stream)
 {
-	return launch1d(cells_pack_kernel, n_owners, 256, stream, dom, n_owners);
+	return launch1d(cells_pack_kernel, n, 256, stream, dom, n);
 }
 
 hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream)

@@ -549,11 +549,75 @@ def test_cells_inactive_and_reactivation():
 
 
 def test_agent_outside_grid_is_an_error():
+    """The recompute runs on the stream without a host wait: the error comes
+    from the next call that waits for it, and the positions stay marked for
+    another recompute."""
     p, ag = sc.simple_cells(3)
     ag.positions[2] = [60.0, 10.0, 10.0]  # on the max face: index n
     d = gpu_domain(_cells_problem(p), field=np.ones(p.shape), agents=ag)
+    d.step(1)
     with pytest.raises(Exception) as e:
-        d.step(1)
+        d.sync()
+    assert "outside" in str(e.value)
+    d.step(1)
+    with pytest.raises(Exception) as e:
+        d.field()
+    assert "outside" in str(e.value)
+
+
+def _hip_copy_to_device(dst_ptr, arr):
+    """Blocking host-to-device copy through the HIP runtime (the domain's stream
+    must be idle: call sync first)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    a = np.ascontiguousarray(arr, dtype=np.float64)
+    assert hip.hipMemcpy(dst_ptr, a.ctypes.data, a.nbytes, 1) == 0  # hipMemcpyHostToDevice
+
+
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+def test_positions_moved_on_device_equal_new_agent_arrays(reduce):
+    """Agents moved by writing the device positions and recomputing give the
+    field of uploading the moved agents, with graphs captured before the move
+    kept (the recompute changes no launch)."""
+    rng = np.random.default_rng(31)
+    p, f0, dirs, bulk, _ = sc.config2(side=36, S=3, n_cells=0, seed=31)
+    ag = _random_agents(p, 4000, rng, clustered=True)
+    moved = np.clip(ag.positions + rng.normal(0, 30, ag.positions.shape), 0,
+                    np.nextafter(np.array([p.nx, p.ny, p.nz]) * 20.0, 0))
+    a = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, reduce=reduce)
+    b = gpu_domain(p, field=f0, dirichlet=dirs, bulk=bulk, agents=ag, reduce=reduce)
+    a.step_graph(10, 5)
+    b.step_graph(10, 5)
+    ag.positions[:] = moved
+    set_agents(a, ag)
+    b.sync()
+    _hip_copy_to_device(b.agent_positions_device_ptr(), moved)
+    b.recompute_positional_data()
+    a.step_graph(10, 5)
+    b.step_graph(10, 5)
+    fa, fb = a.field(), b.field()
+    if reduce == "sorted":
+        assert np.array_equal(fa, fb)
+    else:
+        assert rel_linf(fb, fa.astype(np.float64)).max() < 1e-6
+    sa, sb = a.cell_sums(), b.cell_sums()
+    assert np.array_equal(sa[0], sb[0]) or reduce == "atomic"
+    assert len(sa[0]) == len(sb[0]) and sa[4] == sb[4]
+
+
+def test_positions_moved_on_device_outside_grid_is_reported():
+    p, ag = sc.simple_cells(3)
+    d = gpu_domain(_cells_problem(p), field=np.ones(p.shape), agents=ag)
+    d.step(1)
+    d.sync()
+    bad = ag.positions.copy()
+    bad[1] = [-5.0, 10.0, 10.0]
+    _hip_copy_to_device(d.agent_positions_device_ptr(), bad)
+    d.recompute_positional_data()
+    d.step(1)
+    with pytest.raises(Exception) as e:
+        d.sync()
     assert "outside" in str(e.value)
 
 
