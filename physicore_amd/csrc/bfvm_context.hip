@@ -694,21 +694,20 @@ int recompute(bfvm_ctx* c)
 	TRY_HIP(c, hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int32_t), s));
 	if (c->reduce_mode == BFVM_REDUCE_ATOMIC)
 	{
-		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, true, s));
+		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, c->S, true, s));
 		TRY_HIP(c, launch_cells_ballot(c->ddom.p, n, s));
-		TRY_HIP(c, launch_cells_collect_owners(c->ddom.p, n, s));
-		// the atomics collected the owners in no particular order: sort the
-		// list by voxel so that the per-step apply walks the field in order
-		// (the list order does not change any result).  The list has a slot
-		// per agent; slots past the owner count sort last
+		// the owner list in voxel order (the per-step apply then walks the
+		// field in order; the order changes no result): the agents sorted by
+		// (owner voxel, or V for the others), the owners counted on the device
 		TRY_HIP(c, launch_cells_owner_keys(c->ddom.p, c->keys.p, c->vals.p, n, s));
 		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, n,
 									bits_for((uint64_t)c->V), s));
+		TRY_HIP(c, launch_cells_count_owners(c->keys.p, n, (uint32_t)c->V, c->flags.p, s));
 		TRY_HIP(c, hipMemcpyAsync(c->owners.p, c->vals.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
 	}
 	else
 	{
-		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, false, s));
+		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, c->S, false, s));
 		TRY_HIP(c, launch_cells_make_keys(c->ddom.p, c->keys.p, c->vals.p, n, s));
 		TRY_HIP(c, radix_sort_pairs(c->keys.p, c->vals.p, c->tkeys.p, c->tvals.p, c->hist.p, c->scan_tmp.p, n,
 									bits_for((uint64_t)c->V), s));
@@ -717,7 +716,7 @@ int recompute(bfvm_ctx* c)
 		TRY_HIP(c, exclusive_scan(c->head_scan.p, c->scan_tmp.p, n, s));
 		TRY_HIP(c, launch_cells_segment_reduce(c->ddom.p, c->keys.p, c->vals.p, c->heads.p, c->head_scan.p, n, s));
 	}
-	TRY_HIP(c, launch_cells_pack(c->ddom.p, n, s));
+	TRY_HIP(c, launch_cells_pack(c->ddom.p, n, c->S, c->reduce_mode == BFVM_REDUCE_ATOMIC, s));
 	TRY_HIP(c, launch_cells_agent_slots(c->ddom.p, n, s));
 	c->grid_check_pending = true;
 	return rebuild_overlap(c);
@@ -2032,31 +2031,34 @@ int bfvm_cell_sums(bfvm_ctx* c, int64_t* n_owners, int64_t* owners, double* num,
 		*is_conflict = c->conflict ? 1 : 0;
 	if (!owners)
 		return BFVM_OK;
+	// the packed apply list: owners in voxel order and their full sums
 	const int S = c->S;
-	const int64_t n = c->n_agents;
-	std::vector<int32_t> o((size_t)c->n_owners);
-	std::vector<double> rn((size_t)n * S), rd((size_t)n * S), rf((size_t)n * S);
-	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	if (c->n_owners)
-		TRY_HIP(c, hipMemcpy(o.data(), c->owners.p, o.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-	if (n)
+	const size_t m = (size_t)c->n_owners;
+	std::vector<int32_t> o(m);
+	std::vector<double> pn(m * S), pd(m * S), pf(m * S);
+	if (m)
 	{
-		TRY_HIP(c, hipMemcpy(rn.data(), c->rnum.p, rn.size() * sizeof(double), hipMemcpyDeviceToHost));
-		TRY_HIP(c, hipMemcpy(rd.data(), c->rden.p, rd.size() * sizeof(double), hipMemcpyDeviceToHost));
-		TRY_HIP(c, hipMemcpy(rf.data(), c->rfac.p, rf.size() * sizeof(double), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpy(o.data(), c->owners.p, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpy(pn.data(), c->pnum.p, pn.size() * sizeof(double), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpy(pd.data(), c->pden.p, pd.size() * sizeof(double), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpy(pf.data(), c->pfac.p, pf.size() * sizeof(double), hipMemcpyDeviceToHost));
 	}
-	std::sort(o.begin(), o.end());
-	for (size_t k = 0; k < o.size(); k++)
+	std::vector<size_t> order(m);
+	for (size_t k = 0; k < m; k++)
+		order[k] = k;
+	std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return o[a] < o[b]; });
+	for (size_t k = 0; k < m; k++)
 	{
-		owners[k] = o[k];
+		const size_t j = order[k];
+		owners[k] = o[j];
 		for (int s = 0; s < S; s++)
 		{
 			if (num)
-				num[k * S + s] = rn[(size_t)o[k] * S + s];
+				num[k * S + s] = pn[j * S + s];
 			if (den)
-				den[k * S + s] = rd[(size_t)o[k] * S + s];
+				den[k * S + s] = pd[j * S + s];
 			if (fac)
-				fac[k * S + s] = rf[(size_t)o[k] * S + s];
+				fac[k * S + s] = pf[j * S + s];
 		}
 	}
 	return BFVM_OK;

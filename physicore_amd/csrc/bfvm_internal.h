@@ -343,18 +343,19 @@ hipError_t launch_release_flush(const DomainDev* dom, const int64_t* groups, int
 								int S, hipStream_t stream);
 
 // cell recompute (single domain)
-hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream);
+hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, int S, bool clear_ballots, hipStream_t stream);
 hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stream);
-hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream);
-// packs the apply list from owners[0..flags[2]) (both modes); n: the agent count
-hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, hipStream_t stream);
+// packs the apply list from owners[0..flags[2]) (both modes); n: the agent
+// count; own_terms: the sums hold the other agents' terms only (atomic mode)
+hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, int S, bool own_terms, hipStream_t stream);
 // After the pack: owner_of[i] becomes the slot of agent i's owner (-1 stays).
 hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream);
-// atomic mode: keys/vals = (voxel, owner) of the collected owner list, to sort
-// it into voxel order (radix_sort_pairs); the n slots past the owner count
-// (read on the device) take key V and sort last
+// atomic mode: keys/vals = (voxel, agent) of each owner, key V for the other
+// agents, to sort the owners into voxel order (radix_sort_pairs); then the
+// owner count (flags[2]) from the sorted keys
 hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
 								   hipStream_t stream);
+hipError_t launch_cells_count_owners(const uint32_t* keys, int64_t n, uint32_t V, int32_t* flags, hipStream_t stream);
 // sorted mode: keys/vals are scratch of n entries (sorted in place by radix_sort_pairs)
 hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n,
 								  hipStream_t stream);

@@ -73,12 +73,25 @@ __global__ void step_tick_kernel(const DomainDev* __restrict__ doms, int ndoms, 
 // compute_intermediates + the clearing half of clear_ballots
 // (openmp_solver/src/cell_solver.cpp:22-77); voxel of the agent as
 // cartesian_mesh::voxel_position (reactions-diffusion/biofvm/src/mesh.cpp:37-61).
+// One thread per (agent, substrate), so the per-agent arrays [i*S + s] are read
+// and written coalesced; the substrate-0 thread of each agent bins it.
 __global__ void cells_bin_kernel(const DomainDev* __restrict__ dp, int64_t n, int clear_ballots)
 {
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
 	const DomainDev& d = *dp;
+	const int S = d.S;
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n * S)
+		return;
+	const int64_t i = k / S;
+	const double V = d.voxel_volume, dt = d.dt, vol = d.volumes[i];
+	d.num[k] = d.secretion[k] * d.saturation[k] * dt * vol / V;
+	d.den[k] = (d.uptake[k] + d.secretion[k]) * dt * vol / V;
+	d.fac[k] = d.net_export[k] * dt / V;
+	d.rnum[k] = 0;
+	d.rden[k] = 0;
+	d.rfac[k] = 0;
+	if (k != i * S)
+		return;
 	if (!d.is_active[i])
 	{
 		d.cell_voxel[i] = -1;
@@ -101,18 +114,6 @@ __global__ void cells_bin_kernel(const DomainDev* __restrict__ dp, int64_t n, in
 	}
 	const int64_t v = idx[0] + nn[0] * (idx[1] + nn[1] * idx[2]);
 	d.cell_voxel[i] = (int32_t)v;
-	const int S = d.S;
-	const double V = d.voxel_volume, dt = d.dt, vol = d.volumes[i];
-	for (int s = 0; s < S; s++)
-	{
-		const int64_t k = i * S + s;
-		d.num[k] = d.secretion[k] * d.saturation[k] * dt * vol / V;
-		d.den[k] = (d.uptake[k] + d.secretion[k]) * dt * vol / V;
-		d.fac[k] = d.net_export[k] * dt / V;
-		d.rnum[k] = 0;
-		d.rden[k] = 0;
-		d.rfac[k] = 0;
-	}
 	if (clear_ballots)
 		d.ballot[v] = -1;
 }
@@ -120,7 +121,10 @@ __global__ void cells_bin_kernel(const DomainDev* __restrict__ dp, int64_t n, in
 // ballot_and_sum (cell_solver.cpp:79-129) with wavefront aggregation: lanes
 // holding the same voxel in a run of consecutive lanes elect one CAS and
 // reduce their terms with a segmented scan, so one FP64 atomic per run and
-// substrate reaches memory.
+// substrate reaches memory.  The owner's own terms (and the 1 of the
+// denominator) are not added here: the pack adds them to what the other
+// agents of the voxel accumulated, so a voxel held by one agent costs no
+// atomic at all.
 __global__ __launch_bounds__(256) void cells_ballot_kernel(const DomainDev* __restrict__ dp, int64_t n)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,13 +163,15 @@ __global__ __launch_bounds__(256) void cells_ballot_kernel(const DomainDev* __re
 			d.flags[0] = 1;
 	}
 	const int S = d.S;
+	const bool self = key >= 0 && head && won; // this lane is the owner
+	const bool alone = won && lane == seg;	   // (at the tail) a run of the owner only
 	for (int s = 0; s < S; s++)
 	{
 		double vn = 0, vd = 0, vf = 0;
-		if (key >= 0)
+		if (key >= 0 && !self)
 		{
 			vn = d.num[i * S + s];
-			vd = d.den[i * S + s] + ((head && won) ? 1.0 : 0.0);
+			vd = d.den[i * S + s];
 			vf = d.fac[i * S + s];
 		}
 		for (int off = 1; off < kWave; off <<= 1)
@@ -178,33 +184,13 @@ __global__ __launch_bounds__(256) void cells_ballot_kernel(const DomainDev* __re
 				vf += of;
 			}
 		}
-		if (key >= 0 && tail)
+		if (key >= 0 && tail && !alone)
 		{
 			atomicAdd(&d.rnum[(int64_t)owner * S + s], vn);
 			atomicAdd(&d.rden[(int64_t)owner * S + s], vd);
 			atomicAdd(&d.rfac[(int64_t)owner * S + s], vf);
 		}
 	}
-}
-
-// Appends the owners to the owner list: one atomic per wave on the shared
-// counter (the leader reserves the wave's slots, lanes take their rank).
-__global__ void cells_collect_owners_kernel(const DomainDev* __restrict__ dp, int64_t n)
-{
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	const DomainDev& d = *dp;
-	const bool own = i < n && d.cell_voxel[i] >= 0 && d.owner_of[i] == (int)i;
-	const uint64_t m = __ballot(own);
-	if (m == 0)
-		return;
-	const int leader = __ffsll((unsigned long long)m) - 1;
-	int base = 0;
-	if ((int)__lane_id() == leader)
-		base = atomicAdd(&d.flags[2], __popcll(m));
-	base = __shfl(base, leader);
-	const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-	if (own)
-		d.owners[base + rank] = (int)i;
 }
 
 __global__ void cells_make_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals, int64_t n)
@@ -272,45 +258,66 @@ __global__ void cells_segment_reduce_kernel(const DomainDev* __restrict__ dp, co
 	}
 }
 
-// Packs the apply list in owner-list order: the owner's voxel and its FP64
-// sums, so the per-step kernels read them coalesced.
-// Atomic mode: (voxel, owner agent) of each collected owner, to put the owner
-// list in voxel order (the collection order is the order of the atomics).
-// Slots of the list past the owner count (the list is sized for every agent,
-// the count stays on the device) take the key V, so they sort last.
-__global__ void cells_owner_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals,
-										int64_t n)
+// Atomic mode: (voxel, agent) for the owners and key V for every other agent,
+// so that sorting the n pairs puts the owners first, in voxel order.
+__global__ void cells_owner_keys_kernel(const DomainDev* __restrict__ dp, uint32_t* keys, uint32_t* vals, int64_t n)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n)
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
 		return;
 	const DomainDev& d = *dp;
-	if (k >= d.flags[2])
-	{
-		keys[k] = (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz;
-		vals[k] = 0;
-		return;
-	}
-	const int o = d.owners[k];
-	keys[k] = (uint32_t)d.cell_voxel[o];
-	vals[k] = (uint32_t)o;
+	const int v = d.cell_voxel[i];
+	const bool own = v >= 0 && d.owner_of[i] == (int)i;
+	keys[i] = own ? (uint32_t)v : (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz;
+	vals[i] = (uint32_t)i;
 }
 
-__global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n)
+// The owner count of sorted keys (owners before key V): where the keys below V end.
+__global__ void cells_count_owners_kernel(const uint32_t* __restrict__ keys, int64_t n, uint32_t V, int32_t* flags)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	const DomainDev& d = *dp;
-	if (k >= n || k >= d.flags[2])
+	const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= n)
 		return;
-	const int o = d.owners[k];
+	const bool mine = keys[p] < V;
+	const bool next = p + 1 < n && keys[p + 1] < V;
+	if (mine && !next)
+		flags[2] = (int32_t)(p + 1);
+	else if (p == 0 && !mine)
+		flags[2] = 0;
+}
+
+// Packs the apply list in owner-list order: the owner's voxel and its FP64
+// sums, so the per-step kernels read them coalesced.  One thread per (slot,
+// substrate).  own_terms: the sums hold the other agents' terms only (atomic
+// mode), so the owner's own terms and the 1 of the denominator are added here;
+// otherwise (sorted mode) they are the full sums.
+__global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n, int own_terms)
+{
+	const DomainDev& d = *dp;
 	const int S = d.S;
-	d.pvox[k] = d.cell_voxel[o];
-	d.slot_of[o] = (int32_t)k;
-	for (int s = 0; s < S; s++)
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int64_t k = t / S;
+	if (t >= n * S || k >= d.flags[2])
+		return;
+	const int s = (int)(t - k * S);
+	const int o = d.owners[k];
+	const int64_t q = (int64_t)o * S + s;
+	if (s == 0)
 	{
-		d.pnum[k * S + s] = d.rnum[(int64_t)o * S + s];
-		d.pden[k * S + s] = d.rden[(int64_t)o * S + s];
-		d.pfac[k * S + s] = d.rfac[(int64_t)o * S + s];
+		d.pvox[k] = d.cell_voxel[o];
+		d.slot_of[o] = (int32_t)k;
+	}
+	if (own_terms)
+	{
+		d.pnum[t] = d.num[q] + d.rnum[q];
+		d.pden[t] = (d.den[q] + 1.0) + d.rden[q];
+		d.pfac[t] = d.fac[q] + d.rfac[q];
+	}
+	else
+	{
+		d.pnum[t] = d.rnum[q];
+		d.pden[t] = d.rden[q];
+		d.pfac[t] = d.rfac[q];
 	}
 }
 
@@ -806,9 +813,14 @@ hipError_t launch_cells_owner_keys(const DomainDev* dom, uint32_t* keys, uint32_
 	return launch1d(cells_owner_keys_kernel, n, 256, stream, dom, keys, vals, n);
 }
 
-hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, hipStream_t stream)
+hipError_t launch_cells_count_owners(const uint32_t* keys, int64_t n, uint32_t V, int32_t* flags, hipStream_t stream)
 {
-	return launch1d(cells_pack_kernel, n, 256, stream, dom, n);
+	return launch1d(cells_count_owners_kernel, n, 256, stream, keys, n, V, flags);
+}
+
+hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, int S, bool own_terms, hipStream_t stream)
+{
+	return launch1d(cells_pack_kernel, n * S, 256, stream, dom, n, own_terms ? 1 : 0);
 }
 
 hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream)
@@ -851,19 +863,14 @@ hipError_t launch_release_flush(const DomainDev* dom, const int64_t* groups, int
 	return launch1d(release_flush_kernel, n_groups * S, 256, stream, dom, groups, n_groups, agents);
 }
 
-hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, bool clear_ballots, hipStream_t stream)
+hipError_t launch_cells_bin(const DomainDev* dom, int64_t n, int S, bool clear_ballots, hipStream_t stream)
 {
-	return launch1d(cells_bin_kernel, n, 256, stream, dom, n, clear_ballots ? 1 : 0);
+	return launch1d(cells_bin_kernel, n * S, 256, stream, dom, n, clear_ballots ? 1 : 0);
 }
 
 hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stream)
 {
 	return launch1d(cells_ballot_kernel, n, 256, stream, dom, n);
-}
-
-hipError_t launch_cells_collect_owners(const DomainDev* dom, int64_t n, hipStream_t stream)
-{
-	return launch1d(cells_collect_owners_kernel, n, 256, stream, dom, n);
 }
 
 hipError_t launch_cells_make_keys(const DomainDev* dom, uint32_t* keys, uint32_t* vals, int64_t n, hipStream_t stream)
