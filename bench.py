@@ -185,6 +185,7 @@ def bench_served(count, rate, seed=46, cpu=True):
     t0 = srv.now()
     done = srv.run()
     wall = srv.now() - t0
+    srv.close()
     ttc = [r.finished - r.arrival for r in done]
     out = {"workload": f"{count} requests, Poisson rate {rate}/s, side log-uniform [32,256], S 1-6, "
                         f"0-200k cells, 100-5000 steps, budget 200 GB, chunk 10, seed {seed}",

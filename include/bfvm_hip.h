@@ -295,14 +295,17 @@ int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);
  * that share a kernel, over the concatenated work list of their line tiles;
  * the boundary, bulk and cell stages launch once per substrate count.  Every
  * domain follows the stage order of bfvm_step and gets the same results as
- * stepped alone.  The contexts are borrowed (they must outlive the batch) and
- * stay usable on their own: their streams are ordered before and after each
- * batch step by events.  No counterpart in the reference, whose solvers step
+ * stepped alone.  The contexts are borrowed and stay usable on their own:
+ * their streams are ordered before and after each batch step by events.  A
+ * batch is not stepped, synced or queried after one of its contexts is
+ * destroyed; waiting on its tickets and destroying it stay allowed (neither
+ * touches the contexts).  No counterpart in the reference, whose solvers step
  * one microenvironment per call (openmp_solver.cpp:23-40); this is the
  * ensemble form of that call.
  */
 typedef struct bfvm_batch bfvm_batch;
 int bfvm_batch_create(bfvm_ctx* const* ctxs, int32_t count, bfvm_batch** out);
+/* Waits for the batch's work in flight, then frees it (the contexts stay). */
 void bfvm_batch_destroy(bfvm_batch* batch);
 const char* bfvm_batch_last_error(const bfvm_batch* batch);
 
@@ -310,6 +313,12 @@ const char* bfvm_batch_last_error(const bfvm_batch* batch);
  * HIP graph of graph_steps steps (rebuilt when a domain changes). */
 int bfvm_batch_step(bfvm_batch* batch, int64_t iterations, int64_t graph_steps);
 int bfvm_batch_sync(bfvm_batch* batch);
+
+/* A ticket for the work queued on the batch so far (bfvm_batch_wait blocks
+ * until it has run), so that a caller can queue the next steps before waiting
+ * for the previous ones.  Up to 8 tickets are kept; older ones are refused. */
+int bfvm_batch_mark(bfvm_batch* batch, int64_t* ticket);
+int bfvm_batch_wait(bfvm_batch* batch, int64_t ticket);
 
 /* Launches per step: sweeps and the other stages (after any pending
  * recompute). */

@@ -87,6 +87,8 @@ _SIGS = {
     "bfvm_batch_step": ([C.c_void_p, C.c_int64, C.c_int64], C.c_int),
     "bfvm_batch_sync": ([C.c_void_p], C.c_int),
     "bfvm_batch_info": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "bfvm_batch_mark": ([C.c_void_p, C.POINTER(C.c_int64)], C.c_int),
+    "bfvm_batch_wait": ([C.c_void_p, C.c_int64], C.c_int),
     "bfvm_batch_autotune": ([C.c_void_p, C.c_int64, C.POINTER(Tuning)], C.c_int),
 }
 
@@ -359,7 +361,8 @@ def _tuning_dict(t):
 
 class Batch:
     """Several Domains stepped together (a bfvm_batch).  The domains stay
-    usable on their own; they must outlive the batch."""
+    usable on their own.  Once one of them is closed the batch may only wait
+    on its tickets and be closed."""
 
     def __init__(self, domains):
         self.domains = list(domains)
@@ -390,6 +393,15 @@ class Batch:
 
     def sync(self):
         self._check(lib().bfvm_batch_sync(self._h))
+
+    def mark(self):
+        """Ticket for the work queued so far (wait() blocks until it has run)."""
+        t = C.c_int64()
+        self._check(lib().bfvm_batch_mark(self._h, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        self._check(lib().bfvm_batch_wait(self._h, int(ticket)))
 
     def info(self):
         a, b = C.c_int32(), C.c_int32()

@@ -831,6 +831,19 @@ int flush_releases(bfvm_ctx* c)
 	return BFVM_OK;
 }
 
+// The sweep kernels' LDS limits, set once per device (contexts may be
+// created from several threads).
+hipError_t sweep_kernels_ready(int dev)
+{
+	static std::mutex mu;
+	static std::map<int, hipError_t> done;
+	std::lock_guard<std::mutex> lock(mu);
+	auto it = done.find(dev);
+	if (it == done.end())
+		it = done.emplace(dev, init_sweep_kernels()).first;
+	return it->second;
+}
+
 } // namespace
 
 #define FLUSH_RELEASES(c)               \
@@ -861,6 +874,11 @@ struct bfvm_batch
 	static constexpr int kBatchLanes = 4;
 	hipStream_t lane[kBatchLanes - 1] = {};
 	hipEvent_t ev_fork = nullptr, ev_join[kBatchLanes - 1] = {};
+	// bfvm_batch_mark / bfvm_batch_wait: ticket t completes with
+	// marks[t % kMarkRing] (created on first use)
+	static constexpr int kMarkRing = 8;
+	hipEvent_t marks[kMarkRing] = {};
+	int64_t next_mark = 0;
 
 	struct SweepLaunch
 	{
@@ -927,7 +945,13 @@ struct bfvm_batch
 	}
 	~bfvm_batch()
 	{
+		// the launches in flight read the graph, the tables and the tiles
+		if (stream)
+			(void)hipStreamSynchronize(stream);
 		drop_graph();
+		for (hipEvent_t e : marks)
+			if (e)
+				(void)hipEventDestroy(e);
 		if (ev_in)
 			(void)hipEventDestroy(ev_in);
 		if (ev_out)
@@ -979,12 +1003,17 @@ int batch_prepare(bfvm_batch* b)
 	if (sig == b->signature && b->ddoms.p)
 		return BFVM_OK;
 	b->drop_graph();
-	// launches in flight still read the tables about to be replaced
-	if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess)
-		return b->hip_fail(e, "hipStreamSynchronize");
-	for (bfvm_ctx* c : b->ctx)
-		if (hipError_t e = hipStreamSynchronize(c->stream); e != hipSuccess)
+	// launches in flight still read the tables about to be replaced (a new
+	// batch has none: its first step is ordered after the contexts' work by
+	// events, so it need not wait for them here)
+	if (b->ddoms.p)
+	{
+		if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess)
 			return b->hip_fail(e, "hipStreamSynchronize");
+		for (bfvm_ctx* c : b->ctx)
+			if (hipError_t e = hipStreamSynchronize(c->stream); e != hipSuccess)
+				return b->hip_fail(e, "hipStreamSynchronize");
+	}
 
 	std::vector<DomainDev> h(b->ctx.size());
 	for (size_t i = 0; i < b->ctx.size(); i++)
@@ -1457,7 +1486,7 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	hipError_t e = hipGetDevice(&dev);
 	c->device = dev;
 	if (e == hipSuccess)
-		e = init_sweep_kernels();
+		e = sweep_kernels_ready(dev);
 	if (e == hipSuccess)
 		e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
 	if (e == hipSuccess)
@@ -2272,6 +2301,31 @@ int bfvm_batch_sync(bfvm_batch* b)
 	for (size_t i = 0; i < b->ctx.size(); i++)
 		if (int rc = check_recompute(b->ctx[i]))
 			return b->fail(rc, "domain " + std::to_string(i) + ": " + b->ctx[i]->err);
+	return BFVM_OK;
+}
+
+int bfvm_batch_mark(bfvm_batch* b, int64_t* ticket)
+{
+	if (!b || !ticket)
+		return BFVM_ERR_INVALID;
+	const int k = (int)(b->next_mark % bfvm_batch::kMarkRing);
+	if (!b->marks[k])
+		if (hipError_t e = hipEventCreateWithFlags(&b->marks[k], hipEventDisableTiming); e != hipSuccess)
+			return b->hip_fail(e, "hipEventCreate");
+	if (hipError_t e = hipEventRecord(b->marks[k], b->stream); e != hipSuccess)
+		return b->hip_fail(e, "hipEventRecord");
+	*ticket = b->next_mark++;
+	return BFVM_OK;
+}
+
+int bfvm_batch_wait(bfvm_batch* b, int64_t ticket)
+{
+	if (!b || ticket < 0 || ticket >= b->next_mark)
+		return b ? b->fail(BFVM_ERR_INVALID, "unknown ticket") : BFVM_ERR_INVALID;
+	if (ticket + bfvm_batch::kMarkRing <= b->next_mark - 1)
+		return b->fail(BFVM_ERR_INVALID, "ticket too old (its event was reused)");
+	if (hipError_t e = hipEventSynchronize(b->marks[ticket % bfvm_batch::kMarkRing]); e != hipSuccess)
+		return b->hip_fail(e, "hipEventSynchronize");
 	return BFVM_OK;
 }
 

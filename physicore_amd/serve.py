@@ -7,11 +7,23 @@ resident for the request's lifetime, and steps every admitted request together
 in one batch (physicore_amd.bfvm.Batch: one launch per kernel group per sweep),
 a chunk of steps at a time.  Requests that arrive while others run join at the
 next chunk boundary, so their work overlaps on the device; finished requests
-leave, with their result field and time-to-completion.
+leave with their result field and time-to-completion.
+
+The host stays one chunk ahead of the GPU: it queues chunk i, then waits for
+chunk i-1 (a batch ticket), so the GPU never waits for the host between
+chunks.  New requests are created and uploaded while a chunk runs (their own
+streams), and results leave through pinned snapshots copied while later chunks
+run.
 
 Each domain's arithmetic in a batch is the arithmetic of stepping it alone, so
 a request's result does not depend on what it was batched with (with the
 sorted cell reduction the result is bit-identical).
+
+Failures stay with their request: a malformed request (no substrates, a
+non-positive or non-finite time step, non-finite coefficients, fields or
+rates, an active agent outside its mesh) fails at submit; an error the device
+reports for one request fails that request only; a non-finite result fails
+its request.  cancel() withdraws a waiting or running request.
 """
 import dataclasses
 import time
@@ -21,8 +33,10 @@ import numpy as np
 
 from physicore_amd import bfvm
 
+WAITING, ACTIVE, DONE, FAILED, CANCELLED = "waiting", "active", "done", "failed", "cancelled"
 
-@dataclasses.dataclass
+
+@dataclasses.dataclass(eq=False)
 class Request:
     """One simulation request.  Arrays follow physicore_amd.bfvm.Domain."""
     dims: int
@@ -41,6 +55,8 @@ class Request:
     arrival: float = 0.0                      # seconds, on the server clock
     rid: int = -1
     # filled by the server
+    status: str = WAITING
+    error: Optional[str] = None
     admitted: float = float("nan")
     finished: float = float("nan")
     result: Optional[np.ndarray] = None
@@ -55,11 +71,12 @@ class Request:
 
     def device_bytes(self):
         """Upper estimate of the device memory the request holds: the FP32
-        field, the FP64 factor tables, the agent arrays with their FP64 per-cell
-        and per-owner sums, the voxel ballot and the sort buffers."""
+        field and its snapshot copy, the FP64 factor tables, the agent arrays
+        with their FP64 per-cell and per-owner sums, the voxel ballot and the
+        sort buffers."""
         V, S = self.voxels, self.S
         n = 0 if self.agents is None else len(self.agents["volumes"])
-        b = 4 * V * S + 4 * V                     # field, ballot
+        b = 8 * V * S + 4 * V                     # field, snapshot, ballot
         b += 3 * 3 * 8 * max(self.grid) * S         # b', e, c per axis
         b += n * (8 * self.dims + 8 + 1)            # positions, volumes, activity
         b += n * S * 8 * (7 + 10)                   # rates/internalized + sums
@@ -69,95 +86,357 @@ class Request:
         return b + (1 << 20)
 
 
+def _finite(*arrays):
+    return all(a is None or np.all(np.isfinite(np.asarray(a, dtype=np.float64))) for a in arrays)
+
+
+def validate(r):
+    """Why request r cannot run (None if it can): the checks bfvm_create and
+    the agent binning would fail on, made before any device work, so that a
+    bad request fails alone."""
+    try:
+        S = r.S
+        if S < 1:
+            return "a request needs at least one substrate"
+        if len(r.decay) != S:
+            return "diffusion and decay give different substrate counts"
+        if not (r.dims in (1, 2, 3)) or len(r.grid) != 3 or any(int(g) < 1 for g in r.grid):
+            return "bad mesh"
+        if not (np.isfinite(r.dt) and r.dt > 0):
+            return "the time step must be positive and finite"
+        if not _finite(r.diffusion, r.decay) or np.any(np.asarray(r.diffusion) < 0) or np.any(np.asarray(r.decay) < 0):
+            return "coefficients must be finite and non-negative"
+        if r.steps < 0:
+            return "negative step count"
+        if r.field is not None:
+            if np.asarray(r.field).size != r.voxels * S:
+                return "field size does not match the mesh"
+            if not _finite(r.field):
+                return "non-finite initial field"
+        if r.initial is not None and not _finite(r.initial):
+            return "non-finite initial values"
+        if r.faces is not None and not _finite(r.faces[0]):
+            return "non-finite Dirichlet values"
+        if r.bulk is not None and not _finite(*r.bulk[1:]):
+            return "non-finite bulk rates"
+        a = r.agents
+        if a is not None and len(a["volumes"]):
+            if not _finite(a["positions"], a["secretion"], a["saturation"], a["uptake"], a["net_export"],
+                           a.get("internalized"), a["fraction_released"], a["volumes"]):
+                return "non-finite agent data"
+            pos = np.asarray(a["positions"], dtype=np.float64).reshape(len(a["volumes"]), -1)[:, : r.dims]
+            active = np.asarray(a["is_active"]).astype(bool)
+            box = np.asarray(r.grid[: r.dims], dtype=np.float64) * np.asarray(r.voxel[: r.dims], dtype=np.float64)
+            q = pos[active] / np.asarray(r.voxel[: r.dims], dtype=np.float64)
+            if q.size and (np.any(q < 0) or np.any(q >= np.asarray(r.grid[: r.dims])) or np.any(pos[active] >= box)):
+                return "an active agent lies outside the mesh"
+    except (TypeError, ValueError, KeyError, AttributeError) as e:
+        return f"malformed request: {e}"
+    return None
+
+
+class _Entry:
+    """An admitted request: its domain and the steps queued for it."""
+
+    def __init__(self, req, dom):
+        self.req = req
+        self.dom = dom
+        self.done = 0
+
+
+def _retrieve(server, entry):
+    """Helper-thread job: the request's field, copied out through pinned memory
+    after its last queued chunk (the context's stream is ordered after it),
+    then the domain is freed.  Returns the result array."""
+    d = entry.dom
+    try:
+        with server._device_lock:
+            pinned = bfvm.PinnedField(d.shape)
+            ticket = d.snapshot_begin(pinned)
+        try:
+            d.snapshot_wait(ticket)
+            d.sync()
+            res = np.array(pinned.array, copy=True)
+        finally:
+            with server._device_lock:
+                pinned.close()
+    finally:
+        with server._device_lock:
+            d.close()
+    entry.req.finished = server.now()
+    return res
+
+
+def _create(server, r):
+    """Helper-thread job: request r's domain, created and uploaded."""
+    with server._device_lock:
+        return domain_for(r)
+
+
 class Server:
-    """Continuous-batching request loop over one GPU."""
+    """Continuous-batching request loop over one GPU.
+
+    The calling thread admits requests, forms the batch and queues chunks,
+    staying one chunk ahead of the GPU.  Two helper threads do the host work
+    that would otherwise leave the GPU idle between chunks: one creates and
+    uploads admitted requests, the other copies results out and frees their
+    device memory (a free may wait for the device).  Domains and batches
+    leave through the second one too.  Allocation and freeing on the helper
+    threads never overlap the capture of a batch's graph (one lock: the
+    capture happens in the first step of a new batch)."""
 
     def __init__(self, hbm_budget=200e9, chunk=10, graph=True, clock=time.perf_counter):
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+
         self.budget = float(hbm_budget)
         self.chunk = int(chunk)
         self.graph = bool(graph)
         self.clock = clock
         self.t0 = clock()
         self.waiting = []          # submitted, not admitted (arrival order)
-        self.active = []           # (request, domain, steps done)
-        self.done = []
+        self.active = []           # _Entry: admitted and uploaded, steps still to queue
+        self.done = []             # finished, failed or cancelled requests
         self.in_use = 0.0
         self.updates = 0           # voxel*substrate updates performed
+        self.chunks = 0
+        self._admitting = []       # (request, future of its Domain)
+        self._leaving = []         # (entry, future of its result)
         self._batch = None
         self._batch_key = None
-        self.chunks = 0
+        self._inflight = None      # (batch, ticket) of the last queued chunk
+        self._old_batches = []     # replaced batches, freed once their chunks are waited for
+        self._fresh = False        # the current batch has not stepped yet
+        self._cancel = set()
+        self._device_lock = threading.Lock()
+        self._admit_pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="bfvm-admit")
+        self._retire_pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="bfvm-retire")
 
     def now(self):
         return self.clock() - self.t0
 
+    # ------------------------------------------------------------ requests
     def submit(self, req: Request):
+        """Queues a request.  Over the whole budget: ValueError.  Malformed:
+        the request fails at once (status FAILED, error set), the others are
+        unaffected."""
         if req.device_bytes() > self.budget:
             raise ValueError(f"request {req.rid} needs {req.device_bytes()} bytes, over the budget")
+        why = validate(req)
+        if why is not None:
+            self._finish(req, FAILED, why)
+            return
+        req.status = WAITING
         self.waiting.append(req)
 
-    def _admit(self):
+    def cancel(self, rid):
+        """Withdraws request rid: a waiting one at once, an admitted one at the
+        next chunk boundary (its memory is freed once its queued work ends).
+        Returns False if it has already left."""
+        for r in self.waiting:
+            if r.rid == rid:
+                self.waiting.remove(r)
+                self._finish(r, CANCELLED, None)
+                return True
+        if any(e.req.rid == rid for e in self.active) or any(r.rid == rid for r, _ in self._admitting):
+            self._cancel.add(rid)
+            return True
+        return False
+
+    def _finish(self, r, status, error):
+        r.status = status
+        r.error = error
+        if not np.isfinite(r.finished):
+            r.finished = self.now()
+        self.done.append(r)
+
+    def _free(self, obj):
+        """Closes a domain or batch on the helper thread, after its work."""
+        def close():
+            with self._device_lock:
+                obj.close()
+        self._retire_pool.submit(close)
+
+    def _free_old_batches(self):
+        busy = None if self._inflight is None else self._inflight[0]
+        keep = []
+        for b in self._old_batches:
+            if b is busy:
+                keep.append(b)
+            else:
+                self._free(b)
+        self._old_batches = keep
+
+    # ---------------------------------------------------------------- loop
+    def _admit(self, block=False):
         t = self.now()
         still = []
         for r in self.waiting:
             if r.arrival <= t and self.in_use + r.device_bytes() <= self.budget:
-                d = domain_for(r)
                 r.admitted = self.now()
+                r.status = ACTIVE
                 self.in_use += r.device_bytes()
-                self.active.append([r, d, 0])
+                self._admitting.append((r, self._admit_pool.submit(_create, self, r)))
             else:
                 still.append(r)
         self.waiting = still
-
-    def _retire(self):
+        if block and self._admitting and not self.active:
+            self._admitting[0][1].result()  # nothing to run until one is uploaded
         keep = []
-        for item in self.active:
-            r, d, k = item
-            if k >= r.steps:
-                r.result = d.field()
-                r.finished = self.now()
+        for r, fut in self._admitting:
+            if not fut.done():
+                keep.append((r, fut))
+                continue
+            try:
+                d = fut.result()
+            except Exception as e:  # the device refused this request only
                 self.in_use -= r.device_bytes()
-                self.done.append(r)
-            else:
-                keep.append(item)
-        self.active = keep
+                self._finish(r, FAILED, str(e))
+                continue
+            if r.rid in self._cancel:
+                self._cancel.discard(r.rid)
+                self.in_use -= r.device_bytes()
+                self._finish(r, CANCELLED, None)
+                self._free(d)
+                continue
+            self.active.append(_Entry(r, d))
+        self._admitting = keep
+
+    def _collect(self, block=False):
+        keep = []
+        for e, fut in self._leaving:
+            if not (block or fut.done()):
+                keep.append((e, fut))
+                continue
+            self.in_use -= e.req.device_bytes()
+            try:
+                res = fut.result()
+            except Exception as err:
+                self._finish(e.req, FAILED, str(err))
+                continue
+            if not np.all(np.isfinite(res)):
+                self._finish(e.req, FAILED, "non-finite result")
+                continue
+            e.req.result = res
+            self._finish(e.req, DONE, None)
+        self._leaving = keep
+
+    def _wait(self, inflight):
+        """Waits for a queued chunk; a device error fails the requests it came
+        from (each domain is checked on its own)."""
+        batch, ticket = inflight
+        try:
+            batch.wait(ticket)
+        except bfvm.BfvmError as err:
+            self._isolate_failures(err)
+
+    def _isolate_failures(self, batch_error):
+        """Fails the requests whose domains report an error; if none does, the
+        error belongs to the batch and fails all of them.  Syncing every
+        domain also waits for every queued chunk (each context's stream is
+        ordered after the batch's work), so nothing is in flight afterwards."""
+        failed = []
+        for e in list(self.active):
+            try:
+                e.dom.sync()
+            except bfvm.BfvmError as err:
+                failed.append((e, str(err)))
+        if not failed:
+            failed = [(e, str(batch_error)) for e in self.active]
+        for e, why in failed:
+            self.active.remove(e)
+            self.in_use -= e.req.device_bytes()
+            self._finish(e.req, FAILED, why)
+            self._free(e.dom)
+        self._batch_key = None
+        self._inflight = None
 
     def step_once(self):
-        """Admits what has arrived and fits, runs one chunk of the batch and
-        retires finished requests.  Returns False when nothing is left."""
-        self._admit()
+        """Admits what has arrived and fits, queues one chunk of the batch,
+        then waits for the chunk queued before it and collects finished
+        results.  Returns False when nothing is left."""
+        self._admit(block=True)
+        for e in [e for e in self.active if e.req.rid in self._cancel]:
+            self.active.remove(e)
+            self._cancel.discard(e.req.rid)
+            self.in_use -= e.req.device_bytes()
+            self._finish(e.req, CANCELLED, None)
+            self._free(e.dom)
+        self._collect()
         if not self.active:
-            if not self.waiting:
+            if self._inflight is not None:
+                self._wait(self._inflight)
+                self._inflight = None
+                self._free_old_batches()
+            if not self.waiting and not self._admitting:
+                self._collect(block=True)
                 return False
-            # idle until the next arrival
-            nxt = min(r.arrival for r in self.waiting)
-            delay = nxt - self.now()
-            if delay > 0:
-                time.sleep(min(delay, 0.05))
+            if not self._admitting:
+                # idle until the next arrival
+                delay = min(r.arrival for r in self.waiting) - self.now()
+                if delay > 0:
+                    time.sleep(min(delay, 0.05))
             return True
         # a chunk never runs a request past its step count
-        k = min(self.chunk, min(r.steps - done for r, _, done in self.active))
-        key = tuple(id(d) for _, d, _ in self.active)
+        k = min(self.chunk, min(e.req.steps - e.done for e in self.active))
+        key = tuple(id(e.dom) for e in self.active)
         if key != self._batch_key:
             if self._batch is not None:
-                self._batch.close()
-            self._batch = bfvm.Batch([d for _, d, _ in self.active])
+                self._old_batches.append(self._batch)
+            self._batch = bfvm.Batch([e.dom for e in self.active])
             self._batch_key = key
-        self._batch.step(k, graph_steps=self.chunk if (self.graph and k == self.chunk) else 0)
-        self._batch.sync()
+            self._fresh = True
+        batch = self._batch
+        graph_steps = self.chunk if (self.graph and k == self.chunk) else 0
+        try:
+            if self._fresh and graph_steps:
+                with self._device_lock:  # captures the batch's graph
+                    batch.step(k, graph_steps=graph_steps)
+                self._fresh = False
+            else:
+                batch.step(k, graph_steps=graph_steps)
+            ticket = batch.mark()
+        except bfvm.BfvmError as err:
+            if self._inflight is not None:
+                self._wait(self._inflight)
+                self._inflight = None
+            self._isolate_failures(err)
+            return True
         self.chunks += 1
-        for item in self.active:
-            item[2] += k
-            self.updates += k * item[0].voxels * item[0].S
-        self._retire()
+        for e in list(self.active):
+            e.done += k
+            self.updates += k * e.req.voxels * e.req.S
+            if e.done >= e.req.steps:
+                self.active.remove(e)
+                self._leaving.append((e, self._retire_pool.submit(_retrieve, self, e)))
+        # one chunk queued ahead: wait for the one before this (a failure
+        # found there waits for everything and clears _inflight)
+        previous, self._inflight = self._inflight, (batch, ticket)
+        if previous is not None:
+            self._wait(previous)
+        self._free_old_batches()
         return True
 
     def run(self):
+        """Serves until every submitted request has left; returns the finished
+        ones (failed and cancelled ones are in self.done with their status)."""
         while self.step_once():
             pass
+        if self._inflight is not None:
+            self._wait(self._inflight)
+            self._inflight = None
+        self._collect(block=True)
         if self._batch is not None:
-            self._batch.close()
+            self._old_batches.append(self._batch)
             self._batch = None
-        return self.done
+            self._batch_key = None
+        self._free_old_batches()
+        self._retire_pool.submit(lambda: None).result()  # the frees queued so far
+        return [r for r in self.done if r.status == DONE]
+
+    def close(self):
+        self._admit_pool.shutdown(wait=True)
+        self._retire_pool.shutdown(wait=True)
 
 
 def domain_for(r):

@@ -213,6 +213,7 @@ def test_served_stream_real_sizes():
     for r in reqs:
         srv.submit(r)
     done = srv.run()
+    srv.close()
     assert sorted(r.rid for r in done) == list(range(len(reqs)))
     for r in reqs:
         d = serve.domain_for(r)
