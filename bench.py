@@ -171,7 +171,7 @@ def bench_batched(steps, graph_steps=10, seed=44, cpu=True, compare=True):
     return out
 
 
-def bench_served(count, rate, seed=46, cpu=True):
+def bench_served(count, rate, seed=46, cpu=True, **server_kw):
     """Served stream: Poisson arrivals of requests (cube side log-uniform in
     [32, 256], S 1-6, 0-200k cells, 100-5000 steps), HBM budget 200 GB,
     continuous batching in chunks of 10 steps."""
@@ -179,7 +179,7 @@ def bench_served(count, rate, seed=46, cpu=True):
 
     rng = np.random.default_rng(seed)
     reqs = serve.poisson_stream(rng, count, rate, random_field=False)
-    srv = serve.Server(hbm_budget=200e9, chunk=10)
+    srv = serve.Server(**{"hbm_budget": 200e9, "chunk": 10, **server_kw})
     for r in reqs:
         srv.submit(r)
     t0 = srv.now()
@@ -193,6 +193,29 @@ def bench_served(count, rate, seed=46, cpu=True):
             "value": srv.updates / wall, "unit": "voxel*substrate updates/s",
             "p50_s": serve.percentile(ttc, 50), "p99_s": serve.percentile(ttc, 99),
             "chunks": srv.chunks}
+    # each request alone on the idle GPU (graphs of 10 steps): the floor of its
+    # time-to-completion, and their sum, the stream's work done sequentially
+    alone = {}
+    for r in done:
+        d = serve.domain_for(r)
+        d.sync()
+        t1 = time.perf_counter()
+        g = r.steps - r.steps % 10
+        if g:
+            d.step_graph(g, 10)
+        d.step(r.steps - g)
+        d.sync()
+        alone[r.rid] = time.perf_counter() - t1
+        d.close()
+    slow = [(r.finished - r.arrival) / alone[r.rid] for r in done]
+    out["alone"] = {"sequential_s": float(sum(alone.values())),
+                    "p50_s": serve.percentile(list(alone.values()), 50),
+                    "p99_s": serve.percentile(list(alone.values()), 99),
+                    "max_s": max(alone.values()),
+                    "ttc_over_alone_p50": serve.percentile(slow, 50),
+                    "ttc_over_alone_p99": serve.percentile(slow, 99),
+                    "ttc_of_longest_alone": next(r.finished - r.arrival for r in done
+                                                 if alone[r.rid] == max(alone.values()))}
     if cpu:
         # the oracle per request, one after another: 2 timed steps each, scaled
         # to the request's step count

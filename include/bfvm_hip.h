@@ -319,6 +319,8 @@ int bfvm_batch_sync(bfvm_batch* batch);
  * for the previous ones.  Up to 8 tickets are kept; older ones are refused. */
 int bfvm_batch_mark(bfvm_batch* batch, int64_t* ticket);
 int bfvm_batch_wait(bfvm_batch* batch, int64_t ticket);
+/* Whether a ticket's work has run, without blocking: *done = 1 or 0. */
+int bfvm_batch_query(bfvm_batch* batch, int64_t ticket, int32_t* done);
 
 /* Launches per step: sweeps and the other stages (after any pending
  * recompute). */

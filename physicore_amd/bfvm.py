@@ -89,6 +89,7 @@ _SIGS = {
     "bfvm_batch_info": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "bfvm_batch_mark": ([C.c_void_p, C.POINTER(C.c_int64)], C.c_int),
     "bfvm_batch_wait": ([C.c_void_p, C.c_int64], C.c_int),
+    "bfvm_batch_query": ([C.c_void_p, C.c_int64, C.POINTER(C.c_int32)], C.c_int),
     "bfvm_batch_autotune": ([C.c_void_p, C.c_int64, C.POINTER(Tuning)], C.c_int),
 }
 
@@ -402,6 +403,12 @@ class Batch:
 
     def wait(self, ticket):
         self._check(lib().bfvm_batch_wait(self._h, int(ticket)))
+
+    def query(self, ticket):
+        """Whether ticket's work has run (does not block)."""
+        d = C.c_int32()
+        self._check(lib().bfvm_batch_query(self._h, int(ticket), C.byref(d)))
+        return bool(d.value)
 
     def info(self):
         a, b = C.c_int32(), C.c_int32()

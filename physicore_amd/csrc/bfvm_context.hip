@@ -79,11 +79,110 @@ namespace {
 
 thread_local std::string g_error;
 
+// Device blocks of destroyed contexts and batches.  hipFree waits for the
+// whole device, so a server that frees a finished domain while others step
+// would stall for as long as the queued work takes, once per buffer.  The
+// destroy calls first synchronize the object's streams (after which no queued
+// work reads its buffers) and then hand the blocks to this cache; later
+// allocations on the same device take a cached block of a fitting size.
+// Buffers released any other way still go back through hipFree.
+class BlockCache
+{
+  public:
+	static constexpr size_t kLimit = size_t(48) << 30; // bytes held at most; beyond, hipFree
+
+	// a block of at least `bytes` (already rounded by block_bytes), or nullptr
+	void* take(int dev, size_t bytes)
+	{
+		std::lock_guard<std::mutex> g(m_);
+		const size_t most = bytes >= kLarge ? bytes + bytes / 4 : bytes;
+		for (auto it = blocks_.lower_bound({ dev, bytes }); it != blocks_.end(); ++it)
+		{
+			if (it->first.first != dev || it->first.second > most)
+				break;
+			void* p = it->second;
+			total_ -= it->first.second;
+			blocks_.erase(it);
+			return p;
+		}
+		return nullptr;
+	}
+	void put(int dev, size_t bytes, void* p)
+	{
+		{
+			std::lock_guard<std::mutex> g(m_);
+			if (total_ + bytes <= kLimit)
+			{
+				blocks_.insert({ { dev, bytes }, p });
+				total_ += bytes;
+				return;
+			}
+		}
+		(void)hipFree(p);
+	}
+	// returns every cached block of `dev` to the runtime (after an allocation failed)
+	bool trim(int dev)
+	{
+		std::vector<void*> out;
+		{
+			std::lock_guard<std::mutex> g(m_);
+			for (auto it = blocks_.begin(); it != blocks_.end();)
+				if (it->first.first == dev)
+				{
+					total_ -= it->first.second;
+					out.push_back(it->second);
+					it = blocks_.erase(it);
+				}
+				else
+					++it;
+		}
+		for (void* p : out)
+			(void)hipFree(p);
+		return !out.empty();
+	}
+	// allocation granularity: powers of two below 2 MiB, 2 MiB multiples above
+	static size_t block_bytes(size_t bytes)
+	{
+		if (bytes >= kLarge)
+			return (bytes + kLarge - 1) / kLarge * kLarge;
+		size_t b = 256;
+		while (b < bytes)
+			b <<= 1;
+		return b;
+	}
+
+  private:
+	static constexpr size_t kLarge = size_t(2) << 20;
+	std::mutex m_;
+	std::multimap<std::pair<int, size_t>, void*> blocks_;
+	size_t total_ = 0;
+};
+
+// never destroyed: cached blocks are not freed after the runtime shuts down
+BlockCache& block_cache()
+{
+	static BlockCache* c = new BlockCache;
+	return *c;
+}
+
+// set while a destroy call releases the buffers of an object whose streams
+// it has synchronized
+thread_local bool t_recycle = false;
+
+struct RecycleScope
+{
+	bool prev;
+	explicit RecycleScope(bool on) : prev(t_recycle) { t_recycle = on; }
+	~RecycleScope() { t_recycle = prev; }
+};
+
 template <typename T>
 struct dvec
 {
 	T* p = nullptr;
 	size_t n = 0;
+	size_t cap = 0; // bytes of the block
+	int dev = 0;
 	dvec() = default;
 	dvec(const dvec&) = delete;
 	dvec& operator=(const dvec&) = delete;
@@ -91,9 +190,15 @@ struct dvec
 	void release()
 	{
 		if (p)
-			(void)hipFree(p);
+		{
+			if (t_recycle)
+				block_cache().put(dev, cap, p);
+			else
+				(void)hipFree(p);
+		}
 		p = nullptr;
 		n = 0;
+		cap = 0;
 	}
 	hipError_t alloc(size_t count)
 	{
@@ -102,13 +207,27 @@ struct dvec
 		release();
 		if (count == 0)
 			return hipSuccess;
-		hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+		hipError_t e = hipGetDevice(&dev);
 		if (e != hipSuccess)
-		{
-			p = nullptr;
 			return e;
+		const size_t bytes = BlockCache::block_bytes(count * sizeof(T));
+		p = (T*)block_cache().take(dev, bytes);
+		if (!p)
+		{
+			e = hipMalloc((void**)&p, bytes);
+			if (e == hipErrorOutOfMemory && block_cache().trim(dev))
+			{
+				(void)hipGetLastError();
+				e = hipMalloc((void**)&p, bytes);
+			}
+			if (e != hipSuccess)
+			{
+				p = nullptr;
+				return e;
+			}
 		}
 		n = count;
+		cap = bytes;
 		return hipSuccess;
 	}
 	hipError_t upload(const T* h, size_t count, hipStream_t s)
@@ -731,7 +850,8 @@ int check_recompute(bfvm_ctx* c)
 		return BFVM_OK;
 	c->grid_check_pending = false;
 	int32_t f[4] = { 0, 0, 0, 0 };
-	TRY_HIP(c, hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
+	TRY_HIP(c, hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->n_owners = f[2];
 	c->conflict = f[0] != 0;
 	if (f[1])
@@ -1357,21 +1477,30 @@ int batch_run(bfvm_batch* b, int64_t iterations, int64_t graph_steps)
 				rc = batch_enqueue_step(b, b->dslots.p + k * nd, k == graph_steps - 1 ? (uint32_t)graph_steps : 0);
 			hipGraph_t g = nullptr;
 			hipError_t e = hipStreamEndCapture(b->stream, &g);
-			if (rc)
+			if (!rc && e == hipSuccess)
 			{
-				if (g)
-					(void)hipGraphDestroy(g);
-				return rc;
+				b->graph = g;
+				e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
 			}
-			if (e != hipSuccess)
-				return b->hip_fail(e, "hipStreamEndCapture");
-			b->graph = g;
-			if ((e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
-				return b->hip_fail(e, "hipGraphInstantiate");
-			b->graph_steps = graph_steps;
-			b->graph_valid = true;
+			else if (g)
+				(void)hipGraphDestroy(g);
+			if (rc || e != hipSuccess)
+			{
+				// nothing captured ran; a capture can also be spoiled by work
+				// of another thread, so the steps go out one by one instead
+				// (an error of their own is reported from there)
+				b->drop_graph();
+				(void)hipGetLastError();
+				rc = BFVM_OK;
+				graph_steps = 0;
+			}
+			else
+			{
+				b->graph_steps = graph_steps;
+				b->graph_valid = true;
+			}
 		}
-		for (; done + graph_steps <= iterations; done += graph_steps)
+		for (; graph_steps > 0 && done + graph_steps <= iterations; done += graph_steps)
 			if (hipError_t e = hipGraphLaunch(b->gexec, b->stream); e != hipSuccess)
 				return b->hip_fail(e, "hipGraphLaunch");
 	}
@@ -1531,7 +1660,9 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 		if ((e = c->coef_map.upload(m.data(), m.size(), c->stream)) != hipSuccess)
 			return bail(c->hip_fail(e, "coefficient map"));
 	}
-	if ((e = c->flags.alloc(4)) != hipSuccess || (e = hipMemset(c->flags.p, 0, 4 * sizeof(int32_t))) != hipSuccess)
+	if ((e = c->flags.alloc(4)) != hipSuccess ||
+		(e = hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int32_t), c->stream)) != hipSuccess ||
+		(e = hipStreamSynchronize(c->stream)) != hipSuccess)
 		return bail(c->hip_fail(e, "flags"));
 	int rc = build_tiles(c);
 	if (rc)
@@ -1543,7 +1674,17 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 	return BFVM_OK;
 }
 
-void bfvm_destroy(bfvm_ctx* ctx) { delete ctx; }
+void bfvm_destroy(bfvm_ctx* ctx)
+{
+	if (!ctx)
+		return;
+	// once both streams are idle no queued work reads the buffers (batch steps
+	// are ordered before the context's stream), so they go to the block cache
+	const bool idle = (!ctx->stream || hipStreamSynchronize(ctx->stream) == hipSuccess) &&
+					  (!ctx->copy_stream || hipStreamSynchronize(ctx->copy_stream) == hipSuccess);
+	RecycleScope r(idle);
+	delete ctx;
+}
 
 int bfvm_set_dirichlet_faces(bfvm_ctx* c, const double* values, const uint8_t* conds)
 {
@@ -1703,7 +1844,10 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	{
 		TRY_HIP(c, c->internalized_d.alloc(nS));
 		if (nS)
-			TRY_HIP(c, hipMemset(c->internalized_d.p, 0, nS * sizeof(double)));
+		{
+			TRY_HIP(c, hipMemsetAsync(c->internalized_d.p, 0, nS * sizeof(double), c->stream));
+			TRY_HIP(c, hipStreamSynchronize(c->stream));
+		}
 	}
 	TRY_HIP(c, c->fraction_released.upload(fraction_released, nS, c->stream));
 	TRY_HIP(c, c->volumes.upload(volumes, (size_t)n, c->stream));
@@ -1820,16 +1964,25 @@ int bfvm_step_graph(bfvm_ctx* c, int64_t iterations, int64_t graph_steps)
 			rc = enqueue_step(c, nullptr, c->dslots.p + k, k == graph_steps - 1 ? (uint32_t)graph_steps : 0);
 		hipGraph_t g = nullptr;
 		hipError_t e = hipStreamEndCapture(c->stream, &g);
-		if (rc)
+		if (!rc && e == hipSuccess)
 		{
-			if (g)
-				(void)hipGraphDestroy(g);
-			return rc;
+			c->graph = g;
+			e = hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0);
 		}
-		if (e != hipSuccess)
-			return c->hip_fail(e, "hipStreamEndCapture");
-		c->graph = g;
-		TRY_HIP(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+		else if (g)
+			(void)hipGraphDestroy(g);
+		if (rc || e != hipSuccess)
+		{
+			// nothing captured ran; a capture can also be spoiled by work of
+			// another thread, so the steps go out one by one instead (an
+			// error of their own is reported from there)
+			c->drop_graph();
+			(void)hipGetLastError();
+			for (int64_t k = 0; k < iterations; k++)
+				if ((rc = enqueue_step(c, nullptr)))
+					return rc;
+			return BFVM_OK;
+		}
 		c->graph_steps = graph_steps;
 		c->graph_version = c->version;
 	}
@@ -1990,8 +2143,9 @@ int bfvm_release_internalized(bfvm_ctx* c, int64_t agent)
 	{
 		// positions were written on the device (bfvm_agent_positions_device_ptr)
 		TRY_HIP(c, hipStreamSynchronize(c->stream));
-		TRY_HIP(c, hipMemcpy(c->h_positions.data(), c->positions.p, c->h_positions.size() * sizeof(double),
-							 hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpyAsync(c->h_positions.data(), c->positions.p, c->h_positions.size() * sizeof(double),
+								  hipMemcpyDeviceToHost, c->stream));
+		TRY_HIP(c, hipStreamSynchronize(c->stream));
 		c->h_positions_stale = false;
 	}
 	int64_t v;
@@ -2067,10 +2221,11 @@ int bfvm_cell_sums(bfvm_ctx* c, int64_t* n_owners, int64_t* owners, double* num,
 	std::vector<double> pn(m * S), pd(m * S), pf(m * S);
 	if (m)
 	{
-		TRY_HIP(c, hipMemcpy(o.data(), c->owners.p, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-		TRY_HIP(c, hipMemcpy(pn.data(), c->pnum.p, pn.size() * sizeof(double), hipMemcpyDeviceToHost));
-		TRY_HIP(c, hipMemcpy(pd.data(), c->pden.p, pd.size() * sizeof(double), hipMemcpyDeviceToHost));
-		TRY_HIP(c, hipMemcpy(pf.data(), c->pfac.p, pf.size() * sizeof(double), hipMemcpyDeviceToHost));
+		TRY_HIP(c, hipMemcpyAsync(o.data(), c->owners.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+		TRY_HIP(c, hipMemcpyAsync(pn.data(), c->pnum.p, pn.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+		TRY_HIP(c, hipMemcpyAsync(pd.data(), c->pden.p, pd.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+		TRY_HIP(c, hipMemcpyAsync(pf.data(), c->pfac.p, pf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+		TRY_HIP(c, hipStreamSynchronize(c->stream));
 	}
 	std::vector<size_t> order(m);
 	for (size_t k = 0; k < m; k++)
@@ -2276,8 +2431,12 @@ int bfvm_batch_create(bfvm_ctx* const* ctxs, int32_t count, bfvm_batch** out)
 
 void bfvm_batch_destroy(bfvm_batch* b)
 {
-	if (b && b->stream)
-		(void)hipStreamSynchronize(b->stream);
+	if (!b)
+		return;
+	bool idle = !b->stream || hipStreamSynchronize(b->stream) == hipSuccess;
+	for (hipStream_t l : b->lane)
+		idle = idle && (!l || hipStreamSynchronize(l) == hipSuccess);
+	RecycleScope r(idle);
 	delete b;
 }
 
@@ -2318,14 +2477,41 @@ int bfvm_batch_mark(bfvm_batch* b, int64_t* ticket)
 	return BFVM_OK;
 }
 
-int bfvm_batch_wait(bfvm_batch* b, int64_t ticket)
+namespace {
+int check_ticket(bfvm_batch* b, int64_t ticket)
 {
 	if (!b || ticket < 0 || ticket >= b->next_mark)
 		return b ? b->fail(BFVM_ERR_INVALID, "unknown ticket") : BFVM_ERR_INVALID;
 	if (ticket + bfvm_batch::kMarkRing <= b->next_mark - 1)
 		return b->fail(BFVM_ERR_INVALID, "ticket too old (its event was reused)");
+	return BFVM_OK;
+}
+} // namespace
+
+int bfvm_batch_wait(bfvm_batch* b, int64_t ticket)
+{
+	if (int rc = check_ticket(b, ticket))
+		return rc;
 	if (hipError_t e = hipEventSynchronize(b->marks[ticket % bfvm_batch::kMarkRing]); e != hipSuccess)
 		return b->hip_fail(e, "hipEventSynchronize");
+	return BFVM_OK;
+}
+
+int bfvm_batch_query(bfvm_batch* b, int64_t ticket, int32_t* done)
+{
+	if (!done)
+		return BFVM_ERR_INVALID;
+	if (int rc = check_ticket(b, ticket))
+		return rc;
+	const hipError_t e = hipEventQuery(b->marks[ticket % bfvm_batch::kMarkRing]);
+	if (e == hipErrorNotReady)
+	{
+		*done = 0;
+		return BFVM_OK;
+	}
+	if (e != hipSuccess)
+		return b->hip_fail(e, "hipEventQuery");
+	*done = 1;
 	return BFVM_OK;
 }
 

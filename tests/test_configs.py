@@ -214,7 +214,7 @@ def test_served_stream_real_sizes():
         srv.submit(r)
     done = srv.run()
     srv.close()
-    assert sorted(r.rid for r in done) == list(range(len(reqs)))
+    assert sorted(r.rid for r in done) == list(range(len(reqs))), [(r.rid, r.status, r.error) for r in srv.done]
     for r in reqs:
         d = serve.domain_for(r)
         d.step(r.steps)

@@ -712,3 +712,45 @@ def test_time_sweep_reports_each_axis():
     e = gpu_domain(q, field=np.ones(q.shape))
     with pytest.raises(bfvm.BfvmError):
         e.time_sweep(2, 1)
+
+
+# ------------------------------------------------------------------ reused device blocks
+
+@pytest.mark.parametrize("reduce", ["atomic", "sorted"])
+def test_domain_on_reused_blocks_equals_first_run(reduce):
+    """A destroyed domain's device blocks are kept for later allocations.  A
+    domain built on the blocks of one that held very different values (field
+    near 1e6, large rates, many cells) steps to the same field as the first
+    build of it: nothing read before it is written."""
+    from physicore_amd import serve
+
+    rng = np.random.default_rng(77)
+    req = serve.poisson_stream(rng, 1, 1.0, side_range=(40, 40), s_range=(3, 3), max_cells=3000,
+                               step_range=(5, 5))[0]
+    req.reduce = reduce
+    if req.agents is None:
+        pytest.skip("stream drew no cells")
+    bulk_vox = np.arange(0, req.voxels, 7, dtype=np.int64)
+    req.bulk = (bulk_vox, rng.uniform(0, 1, (len(bulk_vox), req.S)), rng.uniform(0, 1, (len(bulk_vox), req.S)),
+                rng.uniform(0, 1, (len(bulk_vox), req.S)))
+
+    def run(r):
+        d = serve.domain_for(r)
+        d.step(r.steps)
+        f = d.field(np.float64)
+        d.close()
+        return f
+
+    first = run(req)
+    poison = serve.Request(**{**req.__dict__, "times": {}})
+    poison.field = rng.uniform(5e5, 1e6, req.field.shape)
+    poison.agents = {k: (None if v is None else np.array(v, copy=True)) for k, v in req.agents.items()}
+    poison.agents["secretion"] *= 1e4
+    poison.agents["uptake"] *= 1e4
+    for _ in range(2):
+        run(poison)
+        again = run(req)
+        if reduce == "sorted":
+            assert np.array_equal(again, first)
+        else:
+            assert rel_linf(again, first).max() < 1e-6

@@ -37,7 +37,7 @@ def test_results_same_alone_and_in_the_mix():
         srv.submit(r)
     done = srv.run()
     srv.close()
-    assert sorted(r.rid for r in done) == list(range(len(reqs)))
+    assert sorted(r.rid for r in done) == list(range(len(reqs))), [(r.rid, r.status, r.error) for r in srv.done]
     # requests overlapped: some were admitted before others finished
     assert any(a.admitted < b.finished and b.admitted < a.finished
                for a in done for b in done if a.rid < b.rid)
@@ -95,34 +95,24 @@ class _FakeDomain:
         if self.failing():
             raise serve.bfvm.BfvmError(-9, "agent outside the mesh")
 
-    def snapshot_begin(self, pinned):
-        assert not self.closed
-        pinned.array[...] = self.steps
-        return 0
-
-    def snapshot_wait(self, ticket):
-        pass
+    def field(self):
+        self.sync()
+        return np.full(self.shape, self.steps, dtype=np.float32)
 
     def close(self):
         self.closed = True
 
 
-class _FakePinned:
-    def __init__(self, shape):
-        self.array = np.zeros(shape, dtype=np.float32)
-
-    def close(self):
-        pass
-
-
 class _FakeBatch:
     log = []
+    shapes = []
     wait_reports = True  # a chunk's wait reports its domains' errors
 
     def __init__(self, doms):
         assert doms and not any(d.closed for d in doms)
         self.doms = list(doms)
         self.ticks = 0
+        self.queried = set()
         self.closed = False
 
     def step(self, k, graph_steps=0):
@@ -130,6 +120,7 @@ class _FakeBatch:
         for d in self.doms:
             d.steps += k
         _FakeBatch.log.append(len(self.doms))
+        _FakeBatch.shapes.append({d.shape for d in self.doms})
 
     def mark(self):
         self.ticks += 1
@@ -139,6 +130,14 @@ class _FakeBatch:
         assert ticket < self.ticks
         if _FakeBatch.wait_reports and any(d.failing() for d in self.doms if not d.closed):
             raise serve.bfvm.BfvmError(-1, "batch failed")
+
+    def query(self, ticket):
+        # the latest chunk is still running at its first query
+        self.wait(ticket)
+        if ticket == self.ticks - 1 and ticket not in self.queried:
+            self.queried.add(ticket)
+            return False
+        return True
 
     def close(self):
         self.closed = True
@@ -157,10 +156,11 @@ def fake_device(monkeypatch):
         made[r.rid] = _FakeDomain(r, fail.get(r.rid))
         return made[r.rid]
 
-    monkeypatch.setattr(serve, "bfvm", types.SimpleNamespace(Batch=_FakeBatch, PinnedField=_FakePinned,
+    monkeypatch.setattr(serve, "bfvm", types.SimpleNamespace(Batch=_FakeBatch,
                                                              BfvmError=bfvm.BfvmError))
     monkeypatch.setattr(serve, "domain_for", create)
     _FakeBatch.log = []
+    _FakeBatch.shapes = []
     _FakeBatch.wait_reports = True
     return made, fail
 
@@ -328,3 +328,36 @@ def test_cancel_running_request_on_the_device():
         d.step(r.steps)
         assert np.array_equal(d.field(), r.result), r.rid
         d.close()
+
+
+def test_size_classes_step_apart(fake_device):
+    """Small and large requests go to different batches (their own streams),
+    and each still gets exactly its steps."""
+    srv = serve.Server(chunk=3, tiers=(1000,))
+    sides = [4, 12, 4, 12, 4, 4, 12]
+    for i, side in enumerate(sides):
+        srv.submit(_tiny(i, 7 + 2 * i, side=side))
+    assert [srv.tier_of(r) for r in srv.waiting] == [0 if s == 4 else 1 for s in sides]
+    done = srv.run()
+    srv.close()
+    assert len(done) == len(sides)
+    for r in done:
+        assert np.all(r.result == 7 + 2 * r.rid)
+    assert all(len(shapes) == 1 for shapes in _FakeBatch.shapes)
+    assert {next(iter(sh)) for sh in _FakeBatch.shapes} == {(4, 4, 4, 2), (12, 12, 12, 2)}
+
+
+def test_batch_follows_membership_when_objects_are_recycled(fake_device, monkeypatch):
+    """Domains that have left are dropped, so new ones may reuse their object
+    ids: the batch must still be rebuilt whenever its members change (a stale
+    batch would step closed domains, which the stand-in batch refuses)."""
+    monkeypatch.setattr(serve, "domain_for", lambda r: _FakeDomain(r))
+    reqs = [_tiny(i, 2 + i % 3) for i in range(40)]
+    srv = serve.Server(chunk=2, tiers=(), hbm_budget=1.5 * reqs[0].device_bytes())  # one at a time
+    for r in reqs:
+        srv.submit(r)
+    done = srv.run()
+    srv.close()
+    assert len(done) == 40
+    for r in done:
+        assert np.all(r.result == 2 + r.rid % 3)
