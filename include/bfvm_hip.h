@@ -228,10 +228,11 @@ int bfvm_cell_sums(bfvm_ctx* ctx, int64_t* n_owners, int64_t* owners, double* nu
 int bfvm_field_device_ptr(bfvm_ctx* ctx, void** ptr);
 int bfvm_stream(bfvm_ctx* ctx, void** stream);
 
-/* Kernel timing on the context stream with HIP events: runs `steps` steps and
- * reports the average duration in microseconds of each stage launch:
- * out_us[0..5] = x-sweep, y-sweep, z-sweep, interior Dirichlet, bulk, cells
- * (0 when a stage does not run). */
+/* Kernel timing on the context stream with HIP events: runs `steps` steps
+ * (at most 64), queued back to back with an event between launches, and
+ * reports the average duration in microseconds of each stage launch (with the
+ * gap before it): out_us[0..5] = x-sweep, y-sweep, z-sweep, interior
+ * Dirichlet, bulk, cells (0 when a stage does not run). */
 int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 
 /* Kernel timing of one sweep: `launches` back-to-back launches of the sweep

@@ -2313,31 +2313,36 @@ int bfvm_time_stages(bfvm_ctx* c, int64_t steps, double* out_us)
 		if (rc)
 			return rc;
 	}
-	hipEvent_t ev[7];
-	for (auto& e : ev)
-		TRY_HIP(c, hipEventCreate(&e));
-	double acc[6] = { 0, 0, 0, 0, 0, 0 };
+	// the steps are queued back to back, as a run queues them, with events
+	// between the launches; read once the last step has run
+	constexpr int64_t kMaxTimed = 64;
+	steps = std::min(steps, kMaxTimed);
+	std::vector<hipEvent_t> ev((size_t)steps * 7, nullptr);
 	int rc = BFVM_OK;
+	for (auto& e : ev)
+		if (hipError_t err = hipEventCreate(&e); err != hipSuccess)
+		{
+			rc = c->hip_fail(err, "hipEventCreate");
+			break;
+		}
 	for (int64_t k = 0; k < steps && rc == BFVM_OK; k++)
+		rc = enqueue_step(c, ev.data() + k * 7);
+	double acc[6] = { 0, 0, 0, 0, 0, 0 };
+	if (rc == BFVM_OK)
 	{
-		rc = enqueue_step(c, ev);
-		if (rc)
-			break;
-		hipError_t e = hipEventSynchronize(ev[6]);
-		if (e != hipSuccess)
-		{
+		if (hipError_t e = hipEventSynchronize(ev.back()); e != hipSuccess)
 			rc = c->hip_fail(e, "hipEventSynchronize");
-			break;
-		}
-		for (int s = 0; s < 6; s++)
-		{
-			float ms = 0;
-			(void)hipEventElapsedTime(&ms, ev[s], ev[s + 1]);
-			acc[s] += ms * 1000.0;
-		}
+		for (int64_t k = 0; k < steps && rc == BFVM_OK; k++)
+			for (int s = 0; s < 6; s++)
+			{
+				float ms = 0;
+				(void)hipEventElapsedTime(&ms, ev[k * 7 + s], ev[k * 7 + s + 1]);
+				acc[s] += ms * 1000.0;
+			}
 	}
 	for (auto& e : ev)
-		(void)hipEventDestroy(e);
+		if (e)
+			(void)hipEventDestroy(e);
 	if (rc)
 		return rc;
 	for (int s = 0; s < 6; s++)
