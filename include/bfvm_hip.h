@@ -328,8 +328,11 @@ int bfvm_batch_query(bfvm_batch* batch, int64_t ticket, int32_t* done);
 int bfvm_batch_info(bfvm_batch* batch, int32_t* sweep_launches, int32_t* stage_launches);
 
 /* Times `steps` steps of the batch under each candidate tuning (x kernel,
- * y/z pipelining, persistence), applies the fastest to every context and
- * returns it in *chosen (may be NULL).  Advances the domains. */
+ * y/z pipelining, persistent workgroups per CU), applies the fastest to every
+ * context and returns it in *chosen (may be NULL).  The candidates step the
+ * domains' buffers; each domain's field, step count and internalized amounts
+ * are copied aside first and restored afterwards, so the domains are left as
+ * they were (a device copy of each field is held meanwhile). */
 int bfvm_batch_autotune(bfvm_batch* batch, int64_t steps, bfvm_tuning* chosen);
 
 #ifdef __cplusplus

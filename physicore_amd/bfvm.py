@@ -415,7 +415,38 @@ class Batch:
         self._check(lib().bfvm_batch_info(self._h, C.byref(a), C.byref(b)))
         return {"sweep_launches": a.value, "stage_launches": b.value}
 
-    def autotune(self, steps=5):
+    def shape_key(self):
+        """What the tuning choice depends on: each domain's dimensions, grid,
+        substrate count and agent count, in batch order."""
+        return ";".join(f"{d.dims}:{'x'.join(map(str, d.grid))}:{d.S}:{d.n_agents}" for d in self.domains)
+
+    def autotune(self, steps=5, cache=None):
+        """Times the candidate tunings (bfvm_batch_autotune; the domains are
+        left as they were) and applies the fastest to every domain.  cache: a
+        JSON file of earlier choices keyed by shape_key(); a hit applies the
+        stored choice without timing, a miss times and adds it (the file is
+        replaced atomically)."""
+        import json
+        import os
+
+        key = self.shape_key()
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+            if key in table:
+                t = dict(table[key])
+                for d in self.domains:
+                    own = d.tuning()
+                    d.set_tuning(**{**t, "round_nearest": own["round_nearest"]})
+                return t
         t = Tuning()
         self._check(lib().bfvm_batch_autotune(self._h, int(steps), C.byref(t)))
-        return _tuning_dict(t)
+        choice = _tuning_dict(t)
+        if cache:
+            table[key] = {k: v for k, v in choice.items() if k != "round_nearest"}
+            tmp = f"{cache}.{os.getpid()}.tmp"
+            with open(tmp, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return choice

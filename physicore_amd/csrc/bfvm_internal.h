@@ -39,10 +39,18 @@ struct DomainDev
 	const int64_t* dir_vox;
 	const float* dir_val;
 	const uint8_t* dir_cond;
-	// bulk: linear voxel, a = dt*S*T, b = 1 + dt*(U+S) (FP64)
+	// bulk: linear voxel, a = dt*S*T, r = 1 / (1 + dt*(U+S)) (FP64)
 	const int64_t* bulk_vox;
 	const double* bulk_a;
-	const double* bulk_b;
+	const double* bulk_r;
+	// a bulk list with the same rates at every listed voxel (per substrate)
+	// whose voxels form at most one run along each line of the last axis,
+	// when the last sweep applies the bulk stage as it stores its results
+	// (null: the bulk stage runs over the list): a and r per substrate [2*S],
+	// and each column's run [first, end) along the last axis [2*columns],
+	// the column of line f being f / S
+	const double* bulk_u;
+	const int32_t* bulk_run;
 	// bulk and cell lists sharing voxels (rebuilt with either list): the apply
 	// list entry at a bulk voxel or -1, and whether an apply entry's voxel is
 	// in the bulk list (stages_kernel applies such voxels from the bulk side)
@@ -170,6 +178,23 @@ __device__ __forceinline__ LineKey line_key(const DomainDev& d, uint32_t stage, 
 }
 
 __host__ __device__ constexpr uint32_t weyl29(uint32_t i) { return (i * kWeyl) >> 3; }
+
+// Rounding of the bulk stage's results, keyed as the last sweep keys its own:
+// element e lies on line f of the last axis at position p, and its 29 random
+// bits are line_key(d, kRoundBulk, f) ^ weyl29(p), so a last sweep that
+// applies the bulk as it stores (sweep_cols_kernel, kFinBulk) rounds with
+// field_round and gets the same bits.  Round-to-nearest is ties away from
+// zero, as in the sweeps.  The truncated value converts exactly in any FP32
+// rounding mode (FP32 subnormals aside).
+__device__ __forceinline__ float bulk_round(const DomainDev& d, double v, int64_t e)
+{
+	const int64_t stride = d.dims == 3 ? (int64_t)d.ny * d.L : d.dims == 2 ? (int64_t)d.L : (int64_t)d.S;
+	const uint32_t f = (uint32_t)(e % stride), p = (uint32_t)(e / stride);
+	const uint32_t k = d.round_nearest ? (1u << 28) : ((mix32(f ^ stage_seed(d, kRoundBulk)) >> 3) ^ weyl29(p));
+	const uint64_t b = __builtin_bit_cast(uint64_t, v);
+	const bool special = ((b >> 52) & 0x7ff) == 0x7ff;
+	return (float)__builtin_bit_cast(double, special ? b : (b + k) & ~(uint64_t)0x1fffffff);
+}
 
 template <typename ACC>
 __device__ __forceinline__ float field_round(ACC v, LineKey key, uint32_t w)
@@ -307,9 +332,12 @@ hipError_t launch_sweep_pipe(int axis, int nmax, bool faces, const DomainDev* do
 // ---- launchers (enqueue on `stream`, return hipError_t) ----
 // fac_cap: doubles reserved per factor table in LDS (max n*S over the domains).
 // finish: apply face Dirichlet conditions in the last sweep of each domain.
-// faces: use the variant that applies face conditions (register y/z kernels).
+// fin: the instantiation's finishing work in the last sweep: kFinNone,
+// kFinFaces (face conditions), kFinBulk (face conditions, then the bulk of
+// DomainDev::bulk_u on each stored value; FP64 register y/z kernels only).
 // S: substrate count (selects the x-scan instantiation; all its domains share it).
-hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, int S, const DomainDev* doms,
+constexpr int kFinNone = 0, kFinFaces = 1, kFinBulk = 2;
+hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, int fin, int S, const DomainDev* doms,
 						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
 						hipStream_t stream);
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);

@@ -35,8 +35,10 @@ __global__ void dirichlet_interior_kernel(const DomainDev* __restrict__ doms, co
 }
 
 // Bulk supply/uptake (bulk_solver.cpp:12-41): D = (D + dt*S*T) / (1 + dt*(U+S)),
-// with a = dt*S*T and b = 1 + dt*(U+S) precomputed in FP64 on the host.  One
-// thread per (listed voxel, substrate).
+// as D = (D + a) * r with a = dt*S*T and r = 1 / (1 + dt*(U+S)) precomputed in
+// FP64 on the host (within an FP64 ulp of the quotient; the division leaves the
+// per-step path, where its latency chain showed).  One thread per (listed
+// voxel, substrate).
 // Sparse stages (bulk, cell apply, per-agent internalized) run over tiles of
 // up to kSparseTile / S entries of one domain: tile = (dom, first entry, count),
 // one workgroup per tile, so the domain's descriptor is read once into SGPRs.
@@ -45,15 +47,13 @@ __global__ void bulk_kernel(const DomainDev* __restrict__ doms, const TileRef* _
 	const TileRef tr = tiles[blockIdx.x];
 	const DomainDev& d = doms[tr.dom];
 	const int S = d.S;
-	const uint32_t seed = stage_seed(d, kRoundBulk);
-	const bool nearest = d.round_nearest != 0;
 	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 	{
 		const int k = tr.a + t / S, s = t % S;
 		const int64_t q = (int64_t)k * S + s;
 		const int64_t e = d.bulk_vox[k] * S + s;
 		float* D = d.field + e;
-		*D = sr_round(((double)*D + d.bulk_a[q]) / d.bulk_b[q], mix32((uint32_t)e ^ seed), nearest);
+		*D = bulk_round(d, ((double)*D + d.bulk_a[q]) * d.bulk_r[q], e);
 	}
 }
 
@@ -398,14 +398,13 @@ __global__ void stages_kernel(const DomainDev* __restrict__ dom, const TileRef* 
 	if ((int)blockIdx.x < nb)
 	{
 		const TileRef tr = btiles[blockIdx.x];
-		const uint32_t bseed = stage_seed(d, kRoundBulk);
 		for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
 		{
 			const int k = tr.a + t / S, s = t % S;
 			const int64_t q = (int64_t)k * S + s;
 			const int64_t e = d.bulk_vox[k] * S + s;
 			float* D = d.field + e;
-			float v = sr_round(((double)*D + d.bulk_a[q]) / d.bulk_b[q], mix32((uint32_t)e ^ bseed), nearest);
+			float v = bulk_round(d, ((double)*D + d.bulk_a[q]) * d.bulk_r[q], e);
 			const int j = d.bulk_cell[k];
 			if (j >= 0)
 				v = cell_update(d, j, s, (double)v, cseed, fused, nearest);

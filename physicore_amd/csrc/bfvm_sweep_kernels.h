@@ -292,9 +292,14 @@ __device__ __forceinline__ float face_value(const line_faces& f, int k, int n, f
 // (AXIS 0) the columns are the S substrates of a row and outer is the row
 // y + ny*z, so a wave reads 64/S rows with stride S and the caches turn the
 // short stride into whole-line fetches.  Padding steps (i >= n) reload element n-1
-// and store to it before the real value, which is stored last.  FACES: the
-// launch finishes the step for domains whose last axis this is.
-template <int AXIS, int NMAX, typename ACC, bool FACES>
+// and store to it before the real value, which is stored last.  FIN (kFinFaces,
+// kFinBulk): the launch finishes the step for domains whose last axis this is,
+// with the face conditions and, for kFinBulk, the bulk stage of a bulk with
+// uniform rates over one run of voxels per line (DomainDev::bulk_run): each
+// stored value v in the run becomes (v + a_s) * r_s, rounded with the bulk
+// stage's key of its line and position (bulk_round), the arithmetic and bits
+// of bulk_kernel on the stored field.
+template <int AXIS, int NMAX, typename ACC, int FIN>
 __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainDev* __restrict__ doms,
 																   const TileRef* __restrict__ tiles, int ntiles,
 																   int finish, int fac_cap)
@@ -345,10 +350,11 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 		// them are kept live (in SGPRs) across the whole solve.
 		int sn = n - 1, sstride4 = stride4;
 		asm volatile("" : "+s"(sn), "+s"(sstride4));
-		if constexpr (FACES)
+		if constexpr (FIN != kFinNone)
 		{
+			const bool last = finish && AXIS == d.dims - 1;
 			line_faces lf {};
-			if (finish && d.face_mask && AXIS == d.dims - 1)
+			if (last && d.face_mask)
 			{
 				if constexpr (AXIS == 0)
 					lf = faces_of_line(d, 0, 0, outer % d.ny, outer / d.ny, c);
@@ -364,14 +370,47 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 			// afterwards (a lane's later store to a voxel wins; hi outranks them)
 			const bool lohi = lf.lo || lf.hi;
 			const float fv = lf.hi ? lf.hi_v : lf.lo_v;
-			thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
-											[&](int i, float val) {
-												buf_store(rsrc, voff, min(i, sn) * sstride4, lohi ? fv : val);
-											});
-			if (!lf.hi && lf.has_min)
-				buf_store(rsrc, voff, 0, lf.min_v);
-			if (!lf.hi && lf.has_max)
-				buf_store(rsrc, voff, sn * sstride4, lf.max_v);
+			if constexpr (FIN == kFinBulk)
+			{
+				// the bulk stage on each final value, rounded with the bulk
+				// key of its line and position (bulk_round's bits); the
+				// padding stores are overwritten by the real value
+				const double* bu = last ? d.bulk_u : nullptr;
+				const int sub = AXIS == 0 ? c : c % S;
+				const double ba = bu ? bu[sub] : 0.0, br = bu ? bu[S + sub] : 1.0;
+				const int col = f / S;
+				const int r0 = bu ? d.bulk_run[2 * col] : 0;
+				const unsigned rlen = bu ? (unsigned)(d.bulk_run[2 * col + 1] - r0) : 0u;
+				const LineKey kb = line_key(d, kRoundBulk, (uint32_t)f);
+				// the bulk value is formed at every position and picked by a
+				// bit mask: with a conditional the compiler branched per
+				// element and kept one more VGPR per element live
+				auto bulk = [&](int i, float x) -> float {
+					const float r = field_round<double>(((double)x + ba) * br, kb, weyl29(i));
+					const unsigned m = 0u - (unsigned)((unsigned)(i - r0) < rlen);
+					return __builtin_bit_cast(float,
+											  (__builtin_bit_cast(unsigned, r) & m) | (__builtin_bit_cast(unsigned, x) & ~m));
+				};
+				thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+												[&](int i, float val) {
+													buf_store(rsrc, voff, min(i, sn) * sstride4, bulk(i, lohi ? fv : val));
+												});
+				if (!lf.hi && lf.has_min)
+					buf_store(rsrc, voff, 0, bulk(0, lf.min_v));
+				if (!lf.hi && lf.has_max)
+					buf_store(rsrc, voff, sn * sstride4, bulk(sn, lf.max_v));
+			}
+			else
+			{
+				thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+												[&](int i, float val) {
+													buf_store(rsrc, voff, min(i, sn) * sstride4, lohi ? fv : val);
+												});
+				if (!lf.hi && lf.has_min)
+					buf_store(rsrc, voff, 0, lf.min_v);
+				if (!lf.hi && lf.has_max)
+					buf_store(rsrc, voff, sn * sstride4, lf.max_v);
+			}
 		}
 		else
 		{

@@ -14,10 +14,10 @@ namespace bfvm {
 namespace sweep {
 kernel_fn pick_cols_x64a(int nmax, bool faces);
 kernel_fn pick_cols_x64b(int nmax, bool faces);
-kernel_fn pick_cols_y64a(int nmax, bool faces);
-kernel_fn pick_cols_y64b(int nmax, bool faces);
-kernel_fn pick_cols_z64a(int nmax, bool faces);
-kernel_fn pick_cols_z64b(int nmax, bool faces);
+kernel_fn pick_cols_y64a(int nmax, int fin);
+kernel_fn pick_cols_y64b(int nmax, int fin);
+kernel_fn pick_cols_z64a(int nmax, int fin);
+kernel_fn pick_cols_z64b(int nmax, int fin);
 kernel_fn pick_cols_x32(int nmax, bool faces);
 kernel_fn pick_cols_y32(int nmax, bool faces);
 kernel_fn pick_cols_z32(int nmax, bool faces);
@@ -120,8 +120,12 @@ kernel_fn pick_lds(int axis)
 	return sweep_lds_kernel<2, ACC>;
 }
 
-kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
+kernel_fn pick(int axis, int variant, int nmax, bool fp64, int fin, int S)
 {
+	const bool faces = fin != kFinNone;
+	// the bulk-finishing stores exist for the FP64 register y/z kernels only
+	if (fin == kFinBulk && (variant != kSweepReg || !fp64 || axis == 0))
+		return nullptr;
 	if (variant == kSweepScan)
 		return axis == 0 ? pick_xscan(S, nmax, fp64) : nullptr;
 	if (variant == kSweepLds)
@@ -197,8 +201,8 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, bool faces, int S)
 	if (axis == 0)
 		return fp64 ? (lo ? pick_cols_x64a(nmax, faces) : pick_cols_x64b(nmax, faces)) : pick_cols_x32(nmax, faces);
 	if (axis == 1)
-		return fp64 ? (lo ? pick_cols_y64a(nmax, faces) : pick_cols_y64b(nmax, faces)) : pick_cols_y32(nmax, faces);
-	return fp64 ? (lo ? pick_cols_z64a(nmax, faces) : pick_cols_z64b(nmax, faces)) : pick_cols_z32(nmax, faces);
+		return fp64 ? (lo ? pick_cols_y64a(nmax, fin) : pick_cols_y64b(nmax, fin)) : pick_cols_y32(nmax, faces);
+	return fp64 ? (lo ? pick_cols_z64a(nmax, fin) : pick_cols_z64b(nmax, fin)) : pick_cols_z32(nmax, faces);
 }
 } // namespace
 } // namespace sweep
@@ -276,7 +280,7 @@ hipError_t init_sweep_kernels()
 						continue;
 					variant = kSweepRows;
 				}
-				sweep::kernel_fn k = sweep::pick(axis, variant, nmax, p & 1, p & 2, 1);
+				sweep::kernel_fn k = sweep::pick(axis, variant, nmax, p & 1, (p & 2) ? kFinFaces : kFinNone, 1);
 				if (!k)
 					return hipErrorInvalidValue;
 				hipError_t e =
@@ -284,13 +288,23 @@ hipError_t init_sweep_kernels()
 				if (e != hipSuccess)
 					return e;
 			}
+	for (int axis = 1; axis < 3; axis++)
+		for (int nmax = 16; nmax <= kRegMaxN; nmax += 16)
+		{
+			sweep::kernel_fn k = sweep::pick(axis, kSweepReg, nmax, true, kFinBulk, 1);
+			if (!k)
+				return hipErrorInvalidValue;
+			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+			if (e != hipSuccess)
+				return e;
+		}
 	for (int S : { 1, 2, 3, 5, 6, 7 })
 		for (int n = xg_max_n(S) + 1; n <= 2 * xg_max_n(S); n++)
 		{
 			const int nh = xgt_bucket(n, S);
 			if (nh == 0 || (n > xg_max_n(S) + 1 && xgt_bucket(n - 1, S) == nh))
 				continue;
-			sweep::kernel_fn k = sweep::pick(0, kSweepXGT, nh, true, false, S);
+			sweep::kernel_fn k = sweep::pick(0, kSweepXGT, nh, true, kFinNone, S);
 			if (!k)
 				return hipErrorInvalidValue;
 			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
@@ -302,7 +316,7 @@ hipError_t init_sweep_kernels()
 		{
 			if (xg_bucket(nmax, S) != nmax)
 				continue;
-			sweep::kernel_fn k = sweep::pick(0, kSweepXG, nmax, true, false, S);
+			sweep::kernel_fn k = sweep::pick(0, kSweepXG, nmax, true, kFinNone, S);
 			if (!k)
 				return hipErrorInvalidValue;
 			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
@@ -314,7 +328,7 @@ hipError_t init_sweep_kernels()
 		{
 			if (xt_bucket(nmax) != nmax)
 				continue;
-			sweep::kernel_fn k = sweep::pick(0, kSweepXT, nmax, true, false, S);
+			sweep::kernel_fn k = sweep::pick(0, kSweepXT, nmax, true, kFinNone, S);
 			if (!k)
 				return hipErrorInvalidValue;
 			hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
@@ -325,7 +339,7 @@ hipError_t init_sweep_kernels()
 		for (int nmax = twist_bucket(kRegMaxN + 1); nmax <= kRegMaxN; nmax += 8)
 			for (int faces = 0; faces < 2; faces++)
 			{
-				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, faces, 1);
+				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, faces ? kFinFaces : kFinNone, 1);
 				if (!k)
 					return hipErrorInvalidValue;
 				hipError_t e =
@@ -394,7 +408,7 @@ hipError_t launch_sweep_pipe(int axis, int nmax, bool faces, const DomainDev* do
 	return hipGetLastError();
 }
 
-hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, int S, const DomainDev* doms,
+hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, int fin, int S, const DomainDev* doms,
 						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
 						hipStream_t stream)
 {
@@ -402,7 +416,7 @@ hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, bool faces, 
 		return hipSuccess;
 	if (grid <= 0 || grid > ntiles)
 		grid = ntiles;
-	sweep::kernel_fn k = sweep::pick(axis, variant, nmax, fp64, faces, S);
+	sweep::kernel_fn k = sweep::pick(axis, variant, nmax, fp64, fin, S);
 	if (!k)
 		return hipErrorInvalidValue;
 	hipLaunchKernelGGL(k, dim3(grid), dim3(kSweepThreads), lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap);

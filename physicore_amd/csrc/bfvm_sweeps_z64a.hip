@@ -5,11 +5,13 @@
 namespace bfvm {
 namespace sweep {
 
-kernel_fn pick_cols_z64a(int nmax, bool faces)
+kernel_fn pick_cols_z64a(int nmax, int fin)
 {
-#define BFVM_BUCKET(N) \
-	case N:            \
-		return faces ? sweep_cols_kernel<2, N, double, true> : sweep_cols_kernel<2, N, double, false>;
+#define BFVM_BUCKET(N)                                                           \
+	case N:                                                                      \
+		return fin == kFinBulk	  ? sweep_cols_kernel<2, N, double, kFinBulk>  \
+			   : fin == kFinFaces ? sweep_cols_kernel<2, N, double, kFinFaces> \
+								  : sweep_cols_kernel<2, N, double, kFinNone>;
 	switch (nmax)
 	{
 		BFVM_BUCKET(16)

@@ -162,19 +162,43 @@ def test_batch_launch_grouping():
     assert info["stage_launches"] == 0
 
 
-def test_batch_autotune_keeps_results():
+def test_batch_autotune_leaves_domains_as_they_were(tmp_path):
+    """The candidates run on the domains' buffers, whose state is restored:
+    afterwards every field is what it was, and stepping continues from the
+    same step count.  The choice goes to the cache file; a batch of the same
+    shapes takes it from there without timing."""
+    import json
+
     require_gpu()
-    scns = _make(6, 6, extras=False, sides=[16, 24, 32])
+    scns = _make(6, 6, extras=True, sides=[16, 24, 32])
     doms = [_gpu(s) for s in scns]
     ref = [_gpu(s) for s in scns]
+    for d in doms + ref:
+        d.step(2)
+    before = [d.field() for d in doms]
     b = bfvm.Batch(doms)
-    t = b.autotune(steps=2)
+    cache = str(tmp_path / "tuning.json")
+    t = b.autotune(steps=2, cache=cache)
     assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "pipeline", "round_nearest"}
-    # the autotuner advanced the domains by 6 candidates x (2 + 2) steps
+    for k, (d, f) in enumerate(zip(doms, before)):
+        assert np.array_equal(d.field(), f), k
+    b.step(3)
+    b.sync()
     for d in ref:
-        d.step(24)
+        d.step(3)
     for k, (d, r) in enumerate(zip(doms, ref)):
         assert rel_linf(d.field(), r.field()).max() < 1e-5, k
+    table = json.load(open(cache))
+    assert list(table) == [b.shape_key()]
+    # a stored choice is applied as it is
+    table[b.shape_key()] = {"persistent_wgs_per_cu": 1, "force_lds_sweeps": 0, "x_sweep": 2, "pipeline": 1}
+    json.dump(table, open(cache, "w"))
+    again = [_gpu(s) for s in scns]
+    b2 = bfvm.Batch(again)
+    assert b2.autotune(steps=2, cache=cache)["x_sweep"] == 2
+    for d in again:
+        tu = d.tuning()
+        assert (tu["x_sweep"], tu["pipeline"], tu["persistent_wgs_per_cu"]) == (2, 1, 1)
 
 
 def test_batch_rejects_bad_lists():

@@ -754,3 +754,125 @@ def test_domain_on_reused_blocks_equals_first_run(reduce):
             assert np.array_equal(again, first)
         else:
             assert rel_linf(again, first).max() < 1e-6
+
+
+# ------------------------------------------------------------------ uniform bulk in the last sweep
+
+def _bulk_voxels(grid, dims, shape):
+    """Listed voxels: every voxel ("dense"), an ellipsoid filling 0.9 of the
+    mesh along each axis ("ball": one run per line of the last axis, over an
+    eighth of the voxels), that ellipsoid with its middle plane across the last
+    axis removed ("split": two runs on some lines), or one filling 0.3 of each
+    axis ("small": under an eighth of the voxels)."""
+    nx, ny, nz = grid
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    lin = ((z * ny + y) * nx + x).ravel()
+    if shape == "dense":
+        return lin
+    c = [(g - 1) / 2 for g in grid]
+    r = [max((0.15 if shape == "small" else 0.45) * g, 0.5) for g in grid]
+    q = ((x - c[0]) / r[0]) ** 2 + ((y - c[1]) / r[1]) ** 2 + (((z - c[2]) / r[2]) ** 2 if dims == 3 else 0)
+    keep = q.ravel() <= 1.0
+    if shape == "split":
+        last = (z if dims == 3 else y).ravel()
+        keep &= last != int(round(c[dims - 1]))
+    return lin[keep]
+
+
+def _uniform_bulk_workloads(dims, grid, S, seed, shape="dense"):
+    """Two workloads alike but for the bulk: the first lists its voxels with
+    the same rates (the last sweep applies them when each line holds one run),
+    the second the same list with another supply rate at one listed voxel j
+    (the bulk stage applies it).  Faces, cells and field are shared.  Returns
+    (first, second, j)."""
+    from physicore_amd import workloads as wl
+
+    rng = np.random.default_rng(seed)
+    V = int(np.prod(grid))
+    vox = _bulk_voxels(grid, dims, shape)
+    D = 10 ** rng.uniform(0, 4, S)
+    lam = rng.uniform(0.01, 1, S)
+    field = rng.uniform(0, 1, (grid[2], grid[1], grid[0], S))
+    vals = rng.uniform(0, 1, (6, S))
+    conds = (rng.uniform(0, 1, (6, S)) < 0.5).astype(np.uint8)
+    conds[2 * dims:] = 0
+    rates = [np.broadcast_to(rng.uniform(lo, hi, S), (len(vox), S)).copy() for lo, hi in ((0, 1), (0, 1), (0, 2))]
+    n = 40
+    box = np.asarray(grid[:dims], dtype=np.float64) * 10.0
+    agents = dict(positions=rng.uniform(0, 1, (n, dims)) * box, secretion=rng.uniform(0, 5, (n, S)),
+                  saturation=np.ones((n, S)), uptake=rng.uniform(0, 5, (n, S)), net_export=np.zeros((n, S)),
+                  internalized=None, fraction_released=np.zeros((n, S)), volumes=np.full(n, 500.0),
+                  is_active=np.ones(n, dtype=np.uint8))
+    k = int(rng.integers(0, len(vox)))
+    out = []
+    for uniform in (True, False):
+        sup = rates[0].copy()
+        if not uniform:
+            sup[k] *= 1.5
+        out.append(wl.Workload(f"uniform bulk {grid}x{S}", dims, tuple(grid), (10, 10, 10), 0.01, D, lam, 5,
+                               field=field, faces=(vals, conds),
+                               bulk=(vox.astype(np.int64), sup, rates[1], rates[2]), agents=agents))
+    assert V >= len(vox)
+    return out[0], out[1], int(vox[k])
+
+
+@pytest.mark.parametrize("nearest", [0, 1])
+@pytest.mark.parametrize("shape", ["dense", "ball", "split", "small"])
+@pytest.mark.parametrize("dims,grid,S", [(3, (9, 7, 12), 3), (2, (13, 11, 1), 2), (3, (24, 20, 40), 4),
+                                         (3, (6, 5, 250), 1)])
+def test_uniform_bulk_applied_by_last_sweep(dims, grid, S, shape, nearest):
+    """A bulk list with the same rates at every listed voxel, one run of
+    listed voxels per line of the last axis and at least an eighth of the
+    voxels listed (every voxel, a large ball) is applied by the last sweep as
+    it stores its results: no bulk launch follows (a batch of that domain has
+    one stage launch fewer), and one step equals the bulk stage's arithmetic
+    bit for bit at every voxel but the one whose rates differ in the
+    comparison domain.  With two runs on a line, or a small ball, the bulk
+    stage runs."""
+    from physicore_amd import bfvm
+    from physicore_amd import workloads as wl
+
+    wu, wo, j = _uniform_bulk_workloads(dims, grid, S, seed=sum(grid) * 7 + S, shape=shape)
+    a, b = wl.make_domain(wu, reduce="sorted"), wl.make_domain(wo, reduce="sorted")
+    for d in (a, b):
+        d.set_tuning(round_nearest=nearest)
+    ba, bb = bfvm.Batch([a]), bfvm.Batch([b])
+    ia, ib = ba.info(), bb.info()
+    ba.close()
+    bb.close()
+    fused = shape in ("dense", "ball")
+    assert ia["stage_launches"] == ib["stage_launches"] - (1 if fused else 0), (ia, ib)
+    a.step(1)
+    b.step(1)
+    fa = a.field().reshape(-1, S)
+    fb = b.field().reshape(-1, S)
+    keep = np.ones(len(fa), dtype=bool)
+    keep[j] = False
+    assert np.array_equal(fa[keep], fb[keep])
+    assert not np.array_equal(fa[j], fb[j])
+
+
+@pytest.mark.parametrize("shape", ["dense", "ball"])
+@pytest.mark.parametrize("dims,grid,S", [(3, (9, 7, 12), 3), (3, (24, 20, 40), 4), (2, (13, 11, 1), 2)])
+def test_uniform_bulk_in_last_sweep_matches_oracle(dims, grid, S, shape):
+    """Five steps with the bulk applied by the last sweep against the oracle
+    (bulk, faces and cells) at the multi-step bar; a batch of two such domains
+    equals each stepped alone bit for bit."""
+    from physicore_amd import bfvm
+    from physicore_amd import workloads as wl
+
+    w, _, _ = _uniform_bulk_workloads(dims, grid, S, seed=99 + S, shape=shape)
+    a = wl.make_domain(w, reduce="sorted")
+    a.step(w.steps)
+    pair = [wl.make_domain(w, reduce="sorted") for _ in range(2)]
+    bt = bfvm.Batch(pair)
+    bt.step(w.steps)
+    bt.sync()
+    bt.close()
+    for d in pair:
+        assert np.array_equal(d.field(), a.field())
+    p, field, dirs, bulk, ag = sc.oracle_inputs(w)
+    s = orc.Solver(p, field.astype(np.float32).astype(np.float64), dirichlet=dirs, bulk=bulk, agents=ag)
+    s.solve(w.steps)
+    err = rel_linf(a.field(), s.field)
+    assert err.max() < 1e-5, err
