@@ -243,11 +243,12 @@ int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
 int bfvm_time_sweep(bfvm_ctx* ctx, int32_t axis, int64_t launches, double* out_us);
 
 /* Kernel of the x sweep.  BFVM_X_AUTO picks per shape (measured on MI355X):
- * the chunked kernels for S = 4, 8 on rows of 96..200 voxels (and S = 4 up to
- * 32), for S = 3 and (from 96 voxels) S = 5, 6, and for S = 2 on even rows up
- * to 200 voxels; for S = 2 rows of 201..400 voxels the two-lane chunked
- * kernel; the scan kernel for S = 1 and other rows past 200 voxels where it
- * fits (up to 512 voxels, S*ceil(nx/64) <= 32); the strided lines otherwise.
+ * the chunked kernels for S = 4, 8 on rows of 96..200 voxels (16-byte chunks
+ * up to 176 voxels, 4-byte chunks above; S = 4 also up to 32), for S = 3 and
+ * (from 96 voxels) S = 5, 6, and for S = 2 on even rows up to 200 voxels; for
+ * S = 2, 4 and 8 rows of 201..400 voxels the two-lane chunked kernel; the
+ * scan kernel for S = 1 and other rows past 200 voxels where it fits (up to
+ * 512 voxels, S*ceil(nx/64) <= 32); the strided lines otherwise.
  * When the chunked choice does not take the shape (fewer rows than a wave
  * holds, a 1-D domain with faces), the scan kernel runs where it fits.  An
  * explicitly chosen kernel that does not take the shape falls back to the
@@ -260,13 +261,17 @@ typedef enum bfvm_x_sweep
 	BFVM_X_SCAN = 3,	  /* lanes along the row, lane-parallel affine scans */
 	BFVM_X_PIPELINED = 4, /* rows prefetched into LDS while the previous tile is solved
 						   * (FP64, S in {1, 2, 4, 8}, nx*S % 4 == 0) */
-	BFVM_X_CHUNKED = 5	  /* one line per lane in VGPRs, filled and emptied through an
+	BFVM_X_CHUNKED = 5,	  /* one line per lane in VGPRs, filled and emptied through an
 						   * LDS transpose of 32-voxel chunks (FP64): 16-byte accesses
 						   * for S in {4, 8}, 4-byte accesses otherwise (S <= 7); rows
 						   * of up to 200 voxels (176 for S = 1, 192 for S = 5, 7) and
 						   * at least one wave of rows (64/S, even for odd S).  Rows up
-						   * to twice that (S <= 7 but 4) take two lanes per line.
+						   * to twice that take two lanes per line (4-byte chunks).
 						   * Other shapes use the strided lines or LDS tiles */
+	BFVM_X_CHUNKED4 = 6	  /* as BFVM_X_CHUNKED with 4-byte accesses for every S
+						   * (1..8), loads streamed two chunks ahead of the
+						   * elimination; rows past one lane's bucket take two
+						   * lanes per line up to twice that */
 } bfvm_x_sweep;
 
 /* Kernel selection and rounding.  persistent_wgs_per_cu: 0 = the default per

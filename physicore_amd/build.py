@@ -39,8 +39,8 @@ def run(cmd, cwd=None):
 
 HIP_SOURCES = ([f"bfvm_twist_{a}{h}.hip" for a in "yz" for h in "abcd"] +
                [f"bfvm_sweeps_xt{s}{h}.hip" for s in (2, 4, 8) for h in "ab"] +
-               [f"bfvm_sweeps_xg{s}{h}.hip" for s in (1, 2, 3, 5, 6, 7) for h in "ab"] +
-               [f"bfvm_sweeps_xgt{s}.hip" for s in (1, 2, 3, 5, 6, 7)] +
+               [f"bfvm_sweeps_xg{s}{h}.hip" for s in range(1, 9) for h in "ab"] +
+               [f"bfvm_sweeps_xgt{s}.hip" for s in range(1, 9)] +
                [f"bfvm_pipe_x{s}.hip" for s in (1, 2, 4, 8)] + [f"bfvm_pipe_{a}{g}.hip" for a in "yz" for g in range(6)] +
                [f"bfvm_sweeps_{a}64{h}.hip" for a in "xyz" for h in "ab"] +
                ["bfvm_sweeps_x32.hip", "bfvm_sweeps_y32.hip", "bfvm_sweeps_z32.hip",

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <map>
@@ -62,6 +63,13 @@ int default_x_sweep(int nx, int S)
 	// 40.7 / 34.1
 	if (S == 2 && nx > 200 && bfvm::xgt_bucket(nx, S))
 		return BFVM_X_CHUNKED;
+	// S = 4, 8 past 176 voxels: 4-byte chunks, one lane per line up to 200
+	// voxels, two up to 400 (16-byte chunks or scan / 4-byte chunks, us,
+	// back-to-back launches): 200^3 x 4: 56.2 / 52.9; 200^3 x 8: 108.2 / 100.6;
+	// 250^3 x 4: 130.1 / 117.2 (scan); 256^3 x 8: 340.5 / 312.1 (scan).  Below,
+	// the 16-byte kernel: 160^3 x 4: 37.0 / 38.3; 96^3 x 4: 13.1 / 18.2
+	if ((S == 4 || S == 8) && nx > 176 && (bfvm::xg_bucket(nx, S) || bfvm::xgt_bucket(nx, S)))
+		return BFVM_X_CHUNKED4;
 	if (nx > 200 || (S != 4 && S != 8 && nx > bfvm::xg_max_n(S)))
 		return bfvm::xscan_vpl(nx, S) ? BFVM_X_SCAN : BFVM_X_CHUNKED;
 	if (S <= 2 || nx > bfvm::kRegMaxN)
@@ -647,16 +655,16 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepXT;
 				c->nmax[axis] = xt_bucket(n);
 			}
-			else if (axis == 0 && xs == BFVM_X_CHUNKED && xg_bucket(n, c->S) && c->fp64 &&
-					 c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
+			else if (axis == 0 && (xs == BFVM_X_CHUNKED || xs == BFVM_X_CHUNKED4) && xg_bucket(n, c->S) &&
+					 c->fp64 && c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepXG;
 				c->nmax[axis] = xg_bucket(n, c->S);
 			}
 			// rows too long for one lane: two lanes per line (the top and bottom
 			// halves in two waves)
-			else if (axis == 0 && xs == BFVM_X_CHUNKED && xgt_bucket(n, c->S) && c->fp64 &&
-					 c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
+			else if (axis == 0 && (xs == BFVM_X_CHUNKED || xs == BFVM_X_CHUNKED4) && xgt_bucket(n, c->S) &&
+					 c->fp64 && c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepXGT;
 				c->nmax[axis] = xgt_bucket(n, c->S);
@@ -664,7 +672,8 @@ int build_tiles(bfvm_ctx* c)
 			// the scan kernel: asked for, or the default's chunked choice did not
 			// take the shape (too few rows for a wave, or 1-D with faces)
 			else if (axis == 0 && vpl && !(c->dims == 1 && c->face_mask) &&
-					 (xs == BFVM_X_SCAN || (c->tuning.x_sweep == BFVM_X_AUTO && xs == BFVM_X_CHUNKED)))
+					 (xs == BFVM_X_SCAN ||
+					  (c->tuning.x_sweep == BFVM_X_AUTO && (xs == BFVM_X_CHUNKED || xs == BFVM_X_CHUNKED4))))
 			{
 				c->variant[axis] = kSweepScan;
 				c->nmax[axis] = vpl;
@@ -1006,11 +1015,13 @@ struct bfvm_batch
 	hipStream_t stream = nullptr;
 	int num_cu = 256;
 	hipEvent_t ev_in = nullptr, ev_out = nullptr;
-	// the groups of one axis touch disjoint domains: they run on kBatchLanes
-	// streams so that the tail of one launch overlaps the next
-	static constexpr int kBatchLanes = 4;
-	hipStream_t lane[kBatchLanes - 1] = {};
-	hipEvent_t ev_fork = nullptr, ev_join[kBatchLanes - 1] = {};
+	// the groups of one axis touch disjoint domains: they run on `lanes`
+	// streams (BFVM_BATCH_LANES, 1..kMaxBatchLanes, default 4) so that the
+	// tail of one launch overlaps the next
+	static constexpr int kMaxBatchLanes = 16;
+	int lanes = 4;
+	hipStream_t lane[kMaxBatchLanes - 1] = {};
+	hipEvent_t ev_fork = nullptr, ev_join[kMaxBatchLanes - 1] = {};
 	// bfvm_batch_mark / bfvm_batch_wait: ticket t completes with
 	// marks[t % kMarkRing] (created on first use)
 	static constexpr int kMarkRing = 8;
@@ -1050,7 +1061,7 @@ struct bfvm_batch
 		std::unique_ptr<TileLaunch> bulk, cells, internalized;
 		int launches() const { return (dirichlet ? 1 : 0) + (bulk ? 1 : 0) + (cells ? 1 : 0) + (internalized ? 1 : 0); }
 	};
-	StageSet stages[3][kBatchLanes];
+	StageSet stages[3][kMaxBatchLanes];
 	dvec<DomainDev> ddoms;
 	std::vector<DomainDev> hdoms;
 	dvec<DomainDev> dslots; // the captured graph's steps: ddoms with step_offset 0, 1, ...
@@ -1096,7 +1107,7 @@ struct bfvm_batch
 			(void)hipEventDestroy(ev_out);
 		if (ev_fork)
 			(void)hipEventDestroy(ev_fork);
-		for (int k = 0; k < kBatchLanes - 1; k++)
+		for (int k = 0; k < kMaxBatchLanes - 1; k++)
 		{
 			if (ev_join[k])
 				(void)hipEventDestroy(ev_join[k]);
@@ -1259,7 +1270,7 @@ int batch_prepare(bfvm_batch* b)
 	// launch's work is the data it sweeps (the sweeps are memory-bound) plus
 	// the stages of the domains whose last sweep it holds, which follow it on
 	// its lane (a sparse entry costs about 10 swept voxel*substrates)
-	constexpr int K = bfvm_batch::kBatchLanes;
+	const int K = b->lanes;
 	auto stage_work = [&](int i) -> int64_t {
 		const bfvm_ctx* c = b->ctx[i];
 		// owners are at most the agents (their count stays on the device)
@@ -1298,7 +1309,7 @@ int batch_prepare(bfvm_batch* b)
 		for (auto& L : b->sweeps)
 			count += L->axis == axis;
 		const int lanes = std::min(K, count);
-		int64_t load[K] = {};
+		int64_t load[bfvm_batch::kMaxBatchLanes] = {};
 		for (size_t j = 0; j < b->sweeps.size(); j++)
 			if (b->sweeps[j]->axis == axis)
 			{
@@ -2435,7 +2446,7 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [-1, 64]");
 	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
-	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_CHUNKED)
+	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_CHUNKED4)
 		return c->fail(BFVM_ERR_INVALID, "x_sweep must be one of the bfvm_x_sweep values");
 	if (t->pipeline != 0 && t->pipeline != 1)
 		return c->fail(BFVM_ERR_INVALID, "pipeline must be 0 or 1");
@@ -2489,7 +2500,9 @@ int bfvm_batch_create(bfvm_ctx* const* ctxs, int32_t count, bfvm_batch** out)
 		e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
 	if (e == hipSuccess)
 		e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
-	for (int k = 0; k < bfvm_batch::kBatchLanes - 1 && e == hipSuccess; k++)
+	if (const char* env = std::getenv("BFVM_BATCH_LANES"))
+		b->lanes = std::clamp(std::atoi(env), 1, bfvm_batch::kMaxBatchLanes);
+	for (int k = 0; k < b->lanes - 1 && e == hipSuccess; k++)
 	{
 		e = hipStreamCreateWithFlags(&b->lane[k], hipStreamNonBlocking);
 		if (e == hipSuccess)

@@ -280,7 +280,7 @@ constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column
 // and the recurrence state exceed 256 VGPRs.
 constexpr int kXtMaxN = 200;
 int xt_bucket(int n);
-// Row bucket of the 4-byte chunked x kernel (S = 1, 2, 3, 5, 6, 7): n rounded up
+// Row bucket of the 4-byte chunked x kernel (S = 1..8): n rounded up
 // to 16, and 200 for rows of 193..200 unless S is 1, 5 or 7; 0 beyond
 // xg_max_n(S) (the line, its staging, the per-lane offsets and the recurrence
 // state exceed 256 VGPRs).

@@ -34,6 +34,8 @@ kernel_fn pick_xgt3(int nmax);
 kernel_fn pick_xgt5(int nmax);
 kernel_fn pick_xgt6(int nmax);
 kernel_fn pick_xgt7(int nmax);
+kernel_fn pick_xgt4(int nmax);
+kernel_fn pick_xgt8(int nmax);
 kernel_fn pick_xg1a(int nmax);
 kernel_fn pick_xg1b(int nmax);
 kernel_fn pick_xg2a(int nmax);
@@ -46,6 +48,10 @@ kernel_fn pick_xg6a(int nmax);
 kernel_fn pick_xg6b(int nmax);
 kernel_fn pick_xg7a(int nmax);
 kernel_fn pick_xg7b(int nmax);
+kernel_fn pick_xg4a(int nmax);
+kernel_fn pick_xg4b(int nmax);
+kernel_fn pick_xg8a(int nmax);
+kernel_fn pick_xg8b(int nmax);
 kernel_fn pick_twist_y_a(int nmax, bool faces);
 kernel_fn pick_twist_y_b(int nmax, bool faces);
 kernel_fn pick_twist_y_c(int nmax, bool faces);
@@ -153,12 +159,16 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, int fin, int S)
 			return pick_xgt2(nmax);
 		case 3:
 			return pick_xgt3(nmax);
+		case 4:
+			return pick_xgt4(nmax);
 		case 5:
 			return pick_xgt5(nmax);
 		case 6:
 			return pick_xgt6(nmax);
 		case 7:
 			return pick_xgt7(nmax);
+		case 8:
+			return pick_xgt8(nmax);
 		default:
 			return nullptr;
 		}
@@ -176,12 +186,16 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, int fin, int S)
 			return lo ? pick_xg2a(nmax) : pick_xg2b(nmax);
 		case 3:
 			return lo ? pick_xg3a(nmax) : pick_xg3b(nmax);
+		case 4:
+			return lo ? pick_xg4a(nmax) : pick_xg4b(nmax);
 		case 5:
 			return lo ? pick_xg5a(nmax) : pick_xg5b(nmax);
 		case 6:
 			return lo ? pick_xg6a(nmax) : pick_xg6b(nmax);
 		case 7:
 			return lo ? pick_xg7a(nmax) : pick_xg7b(nmax);
+		case 8:
+			return lo ? pick_xg8a(nmax) : pick_xg8b(nmax);
 		default:
 			return nullptr;
 		}
@@ -231,14 +245,14 @@ int xt_bucket(int n)
 
 int xg_bucket(int n, int S)
 {
-	if (S < 1 || S > 7 || S == 4 || n < 1 || n > xg_max_n(S))
+	if (S < 1 || S > 8 || n < 1 || n > xg_max_n(S))
 		return 0;
 	return n > 192 ? kXgMaxN : ((n + 15) / 16) * 16;
 }
 
 int xgt_bucket(int n, int S)
 {
-	if (S < 1 || S > 7 || S == 4 || n <= xg_max_n(S) || n > 2 * xg_max_n(S))
+	if (S < 1 || S > 8 || n <= xg_max_n(S) || n > 2 * xg_max_n(S))
 		return 0;
 	const int h = (n + 1) / 2;
 	const int nh = h > 192 ? kXgMaxN : ((h + 15) / 16) * 16;
@@ -298,7 +312,7 @@ hipError_t init_sweep_kernels()
 			if (e != hipSuccess)
 				return e;
 		}
-	for (int S : { 1, 2, 3, 5, 6, 7 })
+	for (int S = 1; S <= 8; S++)
 		for (int n = xg_max_n(S) + 1; n <= 2 * xg_max_n(S); n++)
 		{
 			const int nh = xgt_bucket(n, S);
@@ -311,7 +325,7 @@ hipError_t init_sweep_kernels()
 			if (e != hipSuccess)
 				return e;
 		}
-	for (int S : { 1, 2, 3, 5, 6, 7 })
+	for (int S = 1; S <= 8; S++)
 		for (int nmax = 16; nmax <= xg_max_n(S); nmax += 8)
 		{
 			if (xg_bucket(nmax, S) != nmax)

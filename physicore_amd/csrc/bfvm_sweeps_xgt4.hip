@@ -1,0 +1,29 @@
+// Instantiations of the two-lane chunked x sweep for 4 substrates, rows of
+// 201..400 voxels (half-row buckets).
+#include "bfvm_xgt_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_xgt4(int nmax)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return sweep_xgt_kernel<N, 4>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(112)
+		BFVM_BUCKET(128)
+		BFVM_BUCKET(144)
+		BFVM_BUCKET(160)
+		BFVM_BUCKET(176)
+		BFVM_BUCKET(192)
+		BFVM_BUCKET(200)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

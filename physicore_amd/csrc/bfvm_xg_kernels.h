@@ -1,7 +1,7 @@
-// bfvm_xg_kernels.h -- x sweep of rows of up to 200 voxels with 1, 2, 3, 5, 6
-// or 7 substrates: register lines filled and emptied through a per-wave LDS
+// bfvm_xg_kernels.h -- x sweep of rows of up to 200 voxels with 1 to 8
+// substrates: register lines filled and emptied through a per-wave LDS
 // transpose with 4-byte accesses (device template; instantiated by
-// bfvm_sweeps_xg{1,2,3,5,6,7}{a,b}.hip).
+// bfvm_sweeps_xg{1..8}{a,b}.hip).
 //
 // The strided register lines read a line with stride S, so one load
 // instruction of a wave touches 64/S rows: a cache line per row for 4*S useful
@@ -11,7 +11,8 @@
 // reads 64 consecutive floats of that chunk (the end of one row's chunk and the
 // start of the next row's when S is odd: two contiguous runs).  The lane
 // offsets of the instructions repeat with a period of P = 32*S / gcd(32*S, 64)
-// instructions (S for odd S, 3 for S = 6, 1 for S = 2), so they are P per-lane registers
+// instructions (S for odd S, 3 for S = 6, 1 for S = 2, 2 for S = 4, 4 for
+// S = 8), so they are P per-lane registers
 // plus scalar offsets.  The staged chunk goes to a per-wave LDS buffer (row
 // pitch 33*S floats: the lines' strided reads of a half-wave hit 32 distinct
 // banks) and each lane reads back its own line.  The last chunk (bucket
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xg_kernel(const DomainDev
 																 int finish, int fac_cap)
 {
 	(void)finish;
-	static_assert(S == 1 || S == 2 || S == 3 || S == 5 || S == 6 || S == 7, "4-byte transposes of 1-3, 5-7 substrates");
+	static_assert(S >= 1 && S <= 8, "4-byte transposes of 1-8 substrates");
 	static_assert(NMAX % 8 == 0 && NMAX <= xg_max_n(S), "row bucket");
 	constexpr int R = xg_rows(S);
 	constexpr int PITCH = 33 * S;				  // floats per row in the LDS buffer

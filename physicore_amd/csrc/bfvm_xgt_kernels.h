@@ -1,5 +1,5 @@
-// bfvm_xgt_kernels.h -- x sweep of rows of 201..400 voxels with 1, 2, 3, 5, 6
-// or 7 substrates: two lanes per line (device template; instantiated by
+// bfvm_xgt_kernels.h -- x sweep of rows of 201..400 voxels with 1 to 8
+// substrates: two lanes per line (device template; instantiated by
 // bfvm_sweeps_xgt*.hip).
 //
 // The 4-byte chunked x kernel (bfvm_xg_kernels.h) holds a whole row in one
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kSweepThreads, 2) void sweep_xgt_kernel(const Domai
 																  int finish, int fac_cap)
 {
 	(void)finish;
-	static_assert(S == 1 || S == 2 || S == 3 || S == 5 || S == 6 || S == 7, "4-byte transposes");
+	static_assert(S >= 1 && S <= 8, "4-byte transposes");
 	static_assert(NH % 16 == 0 || NH == kXgMaxN, "half-row bucket");
 	static_assert(NH > kXgtPrefix && NH <= xg_max_n(S), "half-row bucket");
 	constexpr int R = xg_rows(S);

@@ -67,7 +67,7 @@ SHAPES = [
 # pipelined, chunked transpose), pipelined y/z lines, round-to-nearest stores,
 # and every line solved inside an LDS tile
 VARIANTS = {"default": {}, "xstrided": {"x_sweep": 1}, "xrows": {"x_sweep": 2}, "xscan": {"x_sweep": 3},
-            "xpipe": {"x_sweep": 4}, "xchunk": {"x_sweep": 5}, "pipe": {"pipeline": 1},
+            "xpipe": {"x_sweep": 4}, "xchunk": {"x_sweep": 5}, "xchunk4": {"x_sweep": 6}, "pipe": {"pipeline": 1},
             "nearest": {"round_nearest": 1}, "lds": {"force_lds_sweeps": 1}}
 
 
@@ -127,6 +127,35 @@ def test_chunked_x_sweep_matches_strided(dims, grid, S):
     assert np.array_equal(a.field(), b.field())
 
 
+# the 4-byte chunked kernel asked for every S (x_sweep 6), in particular S = 4
+# and 8 (whose default chunked kernel moves 16-byte chunks): bucket tails,
+# partial last waves (16 rows a wave for S = 4, 8 for S = 8), too few rows
+CHUNK4_SHAPES = [
+    (3, (200, 13, 3), 4), (3, (184, 9, 2), 8), (3, (161, 5, 4), 4), (3, (8, 17, 3), 8),
+    (2, (97, 23, 1), 4), (3, (59, 7, 3), 8), (3, (128, 8, 1), 8), (3, (192, 31, 2), 4),
+    (3, (200, 33, 2), 8), (3, (37, 21, 3), 4), (3, (193, 6, 3), 4), (3, (40, 3, 2), 4),
+    (3, (200, 9, 5), 3), (3, (100, 7, 9), 2), (3, (120, 4, 7), 5),
+]
+
+
+@pytest.mark.parametrize("dims,grid,S", CHUNK4_SHAPES)
+def test_chunked4_x_sweep_matches_strided(dims, grid, S):
+    """The 4-byte chunked kernel for any S: the strided kernel's fields bit for
+    bit over several steps"""
+    rng = np.random.default_rng(hash((dims, grid, S, 6)) % 2**32)
+    D = 10 ** rng.uniform(0, 5, S)
+    lam = rng.uniform(0.01, 1, S)
+    p = orc.Problem(dims, grid, (20, 20, 20), 0.01, D, lam)
+    f0 = rng.uniform(0, 1, p.shape)
+    a = gpu_domain(p, field=f0)
+    b = gpu_domain(p, field=f0)
+    a.set_tuning(x_sweep=1)
+    b.set_tuning(x_sweep=6)
+    a.step(3)
+    b.step(3)
+    assert np.array_equal(a.field(), b.field())
+
+
 # rows longer than one lane holds (past 176/192/200 voxels for S = 1/5,7/others):
 # the two-lane chunked kernel, its bucket edges, partial wave pairs, and the
 # longest rows of each S
@@ -135,9 +164,14 @@ LONG_X_SHAPES = [
     (3, (390, 11, 1), 6), (3, (193, 9, 2), 7), (2, (300, 64, 1), 2), (3, (399, 33, 1), 2),
     (3, (352, 65, 1), 1), (3, (384, 9, 1), 7), (3, (209, 21, 1), 3), (3, (385, 10, 2), 6),
 ]
+# S = 4 and 8 (4-byte chunks, x_sweep 6; S = 8 rows past 256 voxels take it by default)
+LONG_X_SHAPES_48 = [
+    (3, (250, 20, 3), 4), (3, (400, 16, 2), 4), (3, (201, 9, 2), 8), (3, (384, 8, 2), 8),
+    (2, (300, 40, 1), 4), (3, (333, 17, 1), 8), (3, (257, 12, 2), 8), (3, (216, 7, 5), 4),
+]
 
 
-@pytest.mark.parametrize("dims,grid,S", LONG_X_SHAPES)
+@pytest.mark.parametrize("dims,grid,S", LONG_X_SHAPES + LONG_X_SHAPES_48)
 def test_two_lane_x_sweep_matches_oracle(dims, grid, S):
     rng = np.random.default_rng(hash((dims, grid, S, 7)) % 2**32)
     D = 10 ** rng.uniform(0, 5, S)
@@ -146,10 +180,26 @@ def test_two_lane_x_sweep_matches_oracle(dims, grid, S):
     f0 = rng.uniform(0, 1, p.shape)
     want = p.diffuse(f0.astype(np.float32).astype(np.float64))
     d = gpu_domain(p, field=f0)
-    d.set_tuning(x_sweep=5)
+    d.set_tuning(x_sweep=6 if S in (4, 8) else 5)
     d.step(1)
     err = rel_linf(d.field(), want)
     assert err.max() < 2e-6, err
+
+
+@pytest.mark.parametrize("S", [4, 8])
+def test_two_lane_x_sweep_s48_batched_equals_alone(S):
+    from physicore_amd import bfvm
+    rng = np.random.default_rng(9 + S)
+    p = orc.Problem(3, (310, 24, 3), (20, 20, 20), 0.01, list(10 ** rng.uniform(0, 5, S)),
+                    list(rng.uniform(0.01, 1, S)))
+    f0 = rng.uniform(0, 1, p.shape)
+    a, b = gpu_domain(p, field=f0), gpu_domain(p, field=f0)
+    for d in (a, b):
+        d.set_tuning(x_sweep=6)
+    batch = bfvm.Batch([b])
+    a.step(4)
+    batch.step(4)
+    assert np.array_equal(a.field(), b.field())
 
 
 def test_two_lane_x_sweep_batched_equals_alone():
