@@ -282,7 +282,10 @@ typedef enum bfvm_x_sweep
  * 192..256 voxels (FP64, nx*S % 4 == 0) with the pipelined register sweep, in
  * which the next tile of 64 lines streams into LDS while the current one is
  * solved.  round_nearest = 1 rounds FP64 results to the FP32 field to nearest
- * instead of stochastically (see bfvm_precision). */
+ * instead of stochastically (see bfvm_precision).  lines_per_block: lines per
+ * workgroup of the one-lane register kernels (y/z column lines, strided and
+ * chunked x lines): 64, 128 or 256 (one, two or four waves; 0 = 256).  The
+ * results do not depend on it. */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
@@ -290,7 +293,8 @@ typedef struct bfvm_tuning
 	int32_t x_sweep;
 	int32_t pipeline;
 	int32_t round_nearest;
-	int32_t reserved[3];
+	int32_t lines_per_block;
+	int32_t reserved[2];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
 int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);

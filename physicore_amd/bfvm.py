@@ -43,7 +43,7 @@ class DomainDesc(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("force_lds_sweeps", C.c_int32),
                 ("x_sweep", C.c_int32), ("pipeline", C.c_int32), ("round_nearest", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("lines_per_block", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 _LIB = None
@@ -342,22 +342,25 @@ class Domain:
         self._check(lib().bfvm_get_tuning(self._h, C.byref(t)))
         return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu,
                 "force_lds_sweeps": t.force_lds_sweeps, "x_sweep": t.x_sweep,
-                "pipeline": t.pipeline, "round_nearest": t.round_nearest}
+                "pipeline": t.pipeline, "round_nearest": t.round_nearest,
+                "lines_per_block": t.lines_per_block}
 
     def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0, x_sweep=0, pipeline=0,
-                   round_nearest=0):
+                   round_nearest=0, lines_per_block=0):
         t = Tuning()
         t.persistent_wgs_per_cu = int(persistent_wgs_per_cu)
         t.force_lds_sweeps = int(force_lds_sweeps)
         t.x_sweep = int(x_sweep)
         t.pipeline = int(pipeline)
         t.round_nearest = int(round_nearest)
+        t.lines_per_block = int(lines_per_block)
         self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))
 
 
 def _tuning_dict(t):
     return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu, "force_lds_sweeps": t.force_lds_sweeps,
-            "x_sweep": t.x_sweep, "pipeline": t.pipeline, "round_nearest": t.round_nearest}
+            "x_sweep": t.x_sweep, "pipeline": t.pipeline, "round_nearest": t.round_nearest,
+            "lines_per_block": t.lines_per_block}
 
 
 class Batch:

@@ -388,6 +388,7 @@ struct bfvm_ctx
 	int fac_cap[3] = { 0, 0, 0 };
 	int variant[3] = { 0, 0, 0 };
 	int nmax[3] = { 0, 0, 0 };
+	int threads[3] = { kSweepThreads, kSweepThreads, kSweepThreads }; // workgroup size of each sweep
 	size_t lds[3] = { 0, 0, 0 };
 	PipeGeometry pipe[3];
 	bfvm_tuning tuning {};
@@ -570,7 +571,10 @@ void domain_pipe_tiles(int axis, int dom, int nx, int ny, int nz, int S, const P
 		t.push_back({ dom, -1, 0, 0 });
 }
 
-void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, int fac_cap, std::vector<TileRef>& t)
+// `threads`: the launch's workgroup size (sweep_threads), which sets the
+// lines of a one-lane register tile.
+void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S, int fac_cap, std::vector<TileRef>& t,
+				  int threads = kSweepThreads)
 {
 	const int L = nx * S;
 	if (axis == 0 && variant == kSweepScan)
@@ -580,12 +584,12 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 			t.push_back({ dom, r0, std::min(kScanRowsPerTile, rows - r0), 0 });
 		return;
 	}
-	// chunked 4-byte x kernels: the rows of 4 waves (of 2 wave pairs for the
-	// two-lane kernel) from the first row
+	// chunked 4-byte x kernels: the rows of the workgroup's waves (of 2 wave
+	// pairs for the two-lane kernel) from the first row
 	if (axis == 0 && (variant == kSweepXG || variant == kSweepXGT))
 	{
 		const int rows = ny * nz;
-		const int per_tile = xg_rows(S) * (kSweepThreads / 64) / (variant == kSweepXGT ? 2 : 1);
+		const int per_tile = variant == kSweepXGT ? xg_rows(S) * (kSweepThreads / 64) / 2 : xg_rows(S) * (threads / 64);
 		for (int r0 = 0; r0 < rows; r0 += per_tile)
 			t.push_back({ dom, r0, 0, 0 });
 		return;
@@ -604,7 +608,7 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 	{
 		// flattened (outer, column) space: x columns are the S lines of a row
 		const int64_t total = axis == 0 ? (int64_t)ny * nz * S : (int64_t)outer * L;
-		const int per_tile = variant == kSweepTwist ? kTwistColsPerTile : kColsPerTile;
+		const int per_tile = variant == kSweepTwist ? kTwistColsPerTile : threads;
 		for (int64_t f0 = 0; f0 < total; f0 += per_tile)
 			t.push_back({ dom, (int)f0, 0, 0 });
 		return;
@@ -706,12 +710,15 @@ int build_tiles(bfvm_ctx* c)
 			{
 				domain_pipe_tiles(axis, 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, c->pipe[axis], t);
 				c->lds[axis] = c->pipe[axis].lds_bytes;
+				c->threads[axis] = kSweepThreads;
 			}
 			else
 			{
+				c->threads[axis] = sweep_threads(c->variant[axis], c->tuning.lines_per_block);
 				domain_tiles(axis, c->variant[axis], 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S,
-							 c->fac_cap[axis], t);
-				c->lds[axis] = sweep_lds_bytes(axis, c->variant[axis], n, c->S, c->L, c->fac_cap[axis]);
+							 c->fac_cap[axis], t, c->threads[axis]);
+				c->lds[axis] =
+					sweep_lds_bytes(axis, c->variant[axis], n, c->S, c->L, c->fac_cap[axis], c->threads[axis]);
 			}
 			if (c->lds[axis] > (size_t)kMaxLdsBytes)
 				return c->fail(BFVM_ERR_UNSUPPORTED, "grid axis " + std::to_string(axis) + " of length " +
@@ -752,7 +759,8 @@ hipError_t launch_axis_sweep(bfvm_ctx* c, int axis, const DomainDev* dd)
 								 c->fac_cap[axis], c->pipe[axis], sweep_grid(c, axis), c->stream);
 	const int fin = (axis == c->dims - 1 && c->bulk_fused) ? kFinBulk : faces ? kFinFaces : kFinNone;
 	return launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64, fin, c->S, dd, c->tiles[axis].p,
-						c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis), c->stream);
+						c->ntiles[axis], 1, c->fac_cap[axis], c->lds[axis], sweep_grid(c, axis), c->stream,
+						c->threads[axis]);
 }
 
 // One step on the stream: sweeps x, y, z (faces fused into the last), interior
@@ -1034,6 +1042,7 @@ struct bfvm_batch
 		bool fp64 = true, faces = false;
 		bool bulk = false; // the domains' last sweep, applying their uniform bulk
 		int fac_cap = 0, grid = 0, ntiles = 0;
+		int threads = kSweepThreads; // workgroup size
 		size_t lds = 0;
 		int64_t units = 0; // voxel*substrates swept: the launch's share of the axis
 		int lane = 0;	   // stream of the launch among the axis' lanes
@@ -1175,7 +1184,7 @@ int batch_prepare(bfvm_batch* b)
 	b->sweeps.clear();
 	for (int axis = 0; axis < 3; axis++)
 	{
-		std::map<std::tuple<int, int, int, int, int>, std::vector<int>> groups;
+		std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> groups;
 		for (size_t i = 0; i < b->ctx.size(); i++)
 		{
 			const bfvm_ctx* c = b->ctx[i];
@@ -1188,7 +1197,7 @@ int batch_prepare(bfvm_batch* b)
 					? c->S
 					: 0;
 			const int bulk = axis == c->dims - 1 && c->bulk_fused ? 1 : 0;
-			groups[{ v, c->nmax[axis], c->fp64 ? 1 : 0, keyS, bulk }].push_back((int)i);
+			groups[{ v, c->nmax[axis], c->fp64 ? 1 : 0, keyS, bulk, c->threads[axis] }].push_back((int)i);
 		}
 		for (auto& [key, doms] : groups)
 		{
@@ -1198,6 +1207,7 @@ int batch_prepare(bfvm_batch* b)
 			L->nmax = std::get<1>(key);
 			L->fp64 = std::get<2>(key) != 0;
 			L->bulk = std::get<4>(key) != 0;
+			L->threads = std::get<5>(key);
 			L->doms = doms;
 			int maxS = 1;
 			int64_t lines = 0;
@@ -1250,9 +1260,10 @@ int batch_prepare(bfvm_batch* b)
 				for (int i : doms)
 				{
 					const bfvm_ctx* c = b->ctx[i];
-					domain_tiles(axis, L->variant, i, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, L->fac_cap, t);
+					domain_tiles(axis, L->variant, i, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, L->fac_cap, t,
+								 L->threads);
 					L->lds = std::max(L->lds, sweep_lds_bytes(axis, L->variant, (int)c->n[axis], c->S, c->L,
-															  L->fac_cap));
+															  L->fac_cap, L->threads));
 				}
 				if (L->lds > (size_t)kMaxLdsBytes)
 					return b->fail(BFVM_ERR_UNSUPPORTED, "sweep group needs more than 160 KiB of LDS");
@@ -1433,7 +1444,7 @@ int batch_enqueue_step(bfvm_batch* b, const DomainDev* dd = nullptr, uint32_t ti
 			else
 				e = launch_sweep(L->axis, L->variant, L->nmax, L->fp64,
 								 L->bulk ? kFinBulk : L->faces ? kFinFaces : kFinNone, L->S, dd, L->tiles.p,
-								 L->ntiles, 1, L->fac_cap, L->lds, L->grid, st);
+								 L->ntiles, 1, L->fac_cap, L->lds, L->grid, st, L->threads);
 			if (e != hipSuccess)
 				return b->hip_fail(e, "sweep launch");
 		}
@@ -2452,6 +2463,9 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "pipeline must be 0 or 1");
 	if (t->round_nearest != 0 && t->round_nearest != 1)
 		return c->fail(BFVM_ERR_INVALID, "round_nearest must be 0 or 1");
+	if (t->lines_per_block != 0 && t->lines_per_block != 64 && t->lines_per_block != 128 &&
+		t->lines_per_block != 256)
+		return c->fail(BFVM_ERR_INVALID, "lines_per_block must be 0, 64, 128 or 256");
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->tuning = *t;
 	// the kernel variants decide whether a uniform bulk is applied by the last
@@ -2628,14 +2642,15 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	if (!b || steps < 1)
 		return BFVM_ERR_INVALID;
 	// candidate tunings, applied to every context of the batch: the x kernel,
-	// pipelining of y/z, persistence of the register kernels (each context
-	// keeps its own rounding mode)
+	// pipelining of y/z, persistence and lines per workgroup of the register
+	// kernels (each context keeps its own rounding mode)
 	std::vector<bfvm_tuning> cand;
-	auto T = [](int x, int pipeline, int per_cu) {
+	auto T = [](int x, int pipeline, int per_cu, int lines = 0) {
 		bfvm_tuning t {};
 		t.x_sweep = x;
 		t.pipeline = pipeline;
 		t.persistent_wgs_per_cu = per_cu;
+		t.lines_per_block = lines;
 		return t;
 	};
 	cand.push_back(T(BFVM_X_AUTO, 0, 0));
@@ -2648,6 +2663,10 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	cand.push_back(T(BFVM_X_AUTO, 0, 2));
 	cand.push_back(T(BFVM_X_AUTO, 0, 4));
 	cand.push_back(T(BFVM_X_TRANSPOSE, 1, 0));
+	// lines per workgroup of the one-lane register kernels
+	cand.push_back(T(BFVM_X_AUTO, 0, 0, 64));
+	cand.push_back(T(BFVM_X_AUTO, 0, 0, 128));
+	cand.push_back(T(BFVM_X_AUTO, 1, 0, 128));
 	auto apply = [&](bfvm_ctx* c, const bfvm_tuning& t0) {
 		bfvm_tuning t = t0;
 		t.round_nearest = c->tuning.round_nearest;

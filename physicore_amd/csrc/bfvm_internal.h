@@ -298,8 +298,17 @@ __host__ __device__ constexpr int xg_rows(int S) { return S % 2 ? (64 / S) & ~1 
 int xgt_bucket(int n, int S);
 constexpr int kXgtPrefix = 32;
 
-// LDS bytes of a sweep launch: factor tables (fac_cap doubles each) plus the tile.
-size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap);
+// LDS bytes of a sweep launch of `threads` per workgroup: factor tables
+// (fac_cap doubles each) plus the tile.
+size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap, int threads = kSweepThreads);
+// Workgroup size of a sweep variant: `lines_per_block` (64, 128, 256; 0 = 256)
+// for the one-lane register kernels (column lines, chunked x lines), 256 for
+// the others.
+inline int sweep_threads(int variant, int lines_per_block)
+{
+	const bool one_lane = variant == kSweepReg || variant == kSweepXT || variant == kSweepXG;
+	return one_lane && lines_per_block > 0 ? lines_per_block : kSweepThreads;
+}
 // x tiles: 64/S rows; LDS row pitch Lp = L + pad with Lp = S (mod 64), so the
 // 64 lanes (r, s) reading element i of their lines hit distinct banks.
 __host__ __device__ inline int x_row_pitch(int L, int S) { return L + ((S - L % 64) % 64 + 64) % 64; }
@@ -339,7 +348,7 @@ hipError_t launch_sweep_pipe(int axis, int nmax, bool faces, const DomainDev* do
 constexpr int kFinNone = 0, kFinFaces = 1, kFinBulk = 2;
 hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, int fin, int S, const DomainDev* doms,
 						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
-						hipStream_t stream);
+						hipStream_t stream, int threads = kSweepThreads);
 hipError_t launch_dirichlet_interior(const DomainDev* doms, const SparseRef* refs, int64_t n, hipStream_t stream);
 // The per-step sparse stages run one thread per (entry, substrate); `S` is the
 // substrate count of every referenced domain (batched launches group by S).

@@ -259,8 +259,9 @@ int xgt_bucket(int n, int S)
 	return nh <= xg_max_n(S) ? nh : 0;
 }
 
-size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
+size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap, int threads)
 {
+	const size_t waves = (size_t)threads / 64;
 	const size_t fac = (size_t)(2 * fac_cap + 64) * sizeof(double);
 	if (variant == kSweepScan)
 		return 0;
@@ -271,9 +272,9 @@ size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap)
 	if (variant == kSweepTwist)
 		return fac + (size_t)sweep::kTwistExchange * sizeof(double);
 	if (variant == kSweepXT)
-		return fac + (size_t)(kSweepThreads / 64) * sweep::xt_buffer_floats(S) * sizeof(float);
+		return fac + waves * sweep::xt_buffer_floats(S) * sizeof(float);
 	if (variant == kSweepXG)
-		return fac + (size_t)(kSweepThreads / 64) * sweep::xg_buffer_floats(S) * sizeof(float);
+		return fac + waves * sweep::xg_buffer_floats(S) * sizeof(float);
 	if (variant == kSweepXGT)
 		return fac + (size_t)sweep::kXgtExchange * sizeof(double) +
 			   (size_t)(kSweepThreads / 64) * sweep::xg_buffer_floats(S) * sizeof(float);
@@ -424,7 +425,7 @@ hipError_t launch_sweep_pipe(int axis, int nmax, bool faces, const DomainDev* do
 
 hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, int fin, int S, const DomainDev* doms,
 						const TileRef* tiles, int ntiles, int finish, int fac_cap, size_t lds_bytes, int grid,
-						hipStream_t stream)
+						hipStream_t stream, int threads)
 {
 	if (ntiles <= 0)
 		return hipSuccess;
@@ -433,7 +434,10 @@ hipError_t launch_sweep(int axis, int variant, int nmax, bool fp64, int fin, int
 	sweep::kernel_fn k = sweep::pick(axis, variant, nmax, fp64, fin, S);
 	if (!k)
 		return hipErrorInvalidValue;
-	hipLaunchKernelGGL(k, dim3(grid), dim3(kSweepThreads), lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap);
+	if (threads != kSweepThreads && (threads % 64 || threads < 64 || threads > kSweepThreads ||
+									 sweep_threads(variant, threads) != threads))
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds_bytes, stream, doms, tiles, ntiles, finish, fac_cap);
 	return hipGetLastError();
 }
 

@@ -179,7 +179,8 @@ def test_batch_autotune_leaves_domains_as_they_were(tmp_path):
     b = bfvm.Batch(doms)
     cache = str(tmp_path / "tuning.json")
     t = b.autotune(steps=2, cache=cache)
-    assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "pipeline", "round_nearest"}
+    assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "pipeline", "round_nearest",
+                      "lines_per_block"}
     for k, (d, f) in enumerate(zip(doms, before)):
         assert np.array_equal(d.field(), f), k
     b.step(3)

@@ -156,6 +156,53 @@ def test_chunked4_x_sweep_matches_strided(dims, grid, S):
     assert np.array_equal(a.field(), b.field())
 
 
+# workgroups of one, two or four waves for the one-lane register kernels (y/z
+# column lines, strided x lines, both chunked x kernels)
+LPB_SHAPES = [(3, (200, 13, 3), 4), (3, (61, 37, 9), 3), (2, (97, 23, 1), 8), (3, (33, 40, 5), 1),
+              (3, (184, 9, 6), 2), (3, (20, 30, 70), 5)]
+
+
+@pytest.mark.parametrize("x_sweep", [0, 1, 5, 6])
+@pytest.mark.parametrize("dims,grid,S", LPB_SHAPES)
+def test_lines_per_block_same_results(dims, grid, S, x_sweep):
+    """lines_per_block 64 and 128 change only the workgroup size: the fields
+    equal the default's (256) bit for bit over several steps"""
+    rng = np.random.default_rng(hash((dims, grid, S, 64)) % 2**32)
+    D = 10 ** rng.uniform(0, 5, S)
+    lam = rng.uniform(0.01, 1, S)
+    p = orc.Problem(dims, grid, (20, 20, 20), 0.01, D, lam)
+    f0 = rng.uniform(0, 1, p.shape)
+    out = []
+    for lpb in (0, 64, 128):
+        d = gpu_domain(p, field=f0)
+        d.set_tuning(x_sweep=x_sweep, lines_per_block=lpb)
+        d.step(3)
+        out.append(d.field())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_lines_per_block_in_a_batch():
+    """A batch whose domains use different workgroup sizes equals each domain
+    stepped alone bit for bit."""
+    from physicore_amd import bfvm
+    rng = np.random.default_rng(64)
+    ps = [orc.Problem(3, g, (20, 20, 20), 0.01, list(10 ** rng.uniform(0, 5, S)), list(rng.uniform(0.01, 1, S)))
+          for g, S in (((40, 30, 20), 4), ((64, 17, 33), 3), ((48, 48, 16), 4), ((100, 12, 9), 2))]
+    f0 = [rng.uniform(0, 1, p.shape) for p in ps]
+    alone = [gpu_domain(p, field=f) for p, f in zip(ps, f0)]
+    mixed = [gpu_domain(p, field=f) for p, f in zip(ps, f0)]
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        lpb = (0, 64, 128, 64)[k]
+        a.set_tuning(lines_per_block=lpb)
+        m.set_tuning(lines_per_block=lpb)
+        a.step(4)
+    b = bfvm.Batch(mixed)
+    b.step(4)
+    b.sync()
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.field(), m.field())
+
+
 # rows longer than one lane holds (past 176/192/200 voxels for S = 1/5,7/others):
 # the two-lane chunked kernel, its bucket edges, partial wave pairs, and the
 # longest rows of each S
