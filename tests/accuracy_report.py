@@ -77,10 +77,24 @@ def main():
         cases.append((f"config3 domain {k} {scn[0].nx}x{scn[0].ny}x{scn[0].nz}x{scn[0].S}", scn[:5],
                       20 if q else 500))
     cases.append(("config4 400x400x100x2, 2e6 cells", sc.config4(), 3 if q else 200))
-    for name, scn, steps in cases:
+    # the served stream of the benchmark (seed 46): its requests of side <= 96
+    # (the oracle finishes them in seconds) at their own step counts
+    from physicore_amd import serve
+    from physicore_amd import workloads as wl
+
+    reqs = serve.poisson_stream(np.random.default_rng(46), 32, 8.0, random_field=False)
+    for r in [r for r in reqs if r.grid[0] <= 96][:3]:
+        w = serve.workload_of(r)
+        cases.append((f"config5 request {r.rid} {r.grid[0]}^3x{r.S}, {w.n_agents} cells", sc.oracle_inputs(w),
+                      min(r.steps, 50) if q else r.steps))
+    refw = wl.reference_benchmark()
+    cases.append(("reference benchmark 250^3x4, 2e6+1 agents, internalized", sc.oracle_inputs(refw),
+                  10 if q else 100, True))
+    for name, scn, steps, *internalized in cases:
         if a.only in name:
             tunings = [json.loads(t) for t in a.tuning] or [None]
-            print(json.dumps(run_case(name, scn, steps, tunings=tunings)), flush=True)
+            print(json.dumps(run_case(name, scn, steps, internalized=bool(internalized and internalized[0]),
+                                      tunings=tunings)), flush=True)
 
 
 if __name__ == "__main__":
