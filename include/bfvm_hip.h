@@ -268,10 +268,14 @@ typedef enum bfvm_x_sweep
 						   * at least one wave of rows (64/S, even for odd S).  Rows up
 						   * to twice that take two lanes per line (4-byte chunks).
 						   * Other shapes use the strided lines or LDS tiles */
-	BFVM_X_CHUNKED4 = 6	  /* as BFVM_X_CHUNKED with 4-byte accesses for every S
+	BFVM_X_CHUNKED4 = 6,  /* as BFVM_X_CHUNKED with 4-byte accesses for every S
 						   * (1..8), loads streamed two chunks ahead of the
 						   * elimination; rows past one lane's bucket take two
 						   * lanes per line up to twice that */
+	BFVM_X_TWO_LANE = 7	  /* two lanes per line (4-byte chunked halves, twisted
+						   * elimination) for rows of 65 voxels up to twice the
+						   * one-lane limit (FP64): twice the waves of the one-lane
+						   * kernels, half the registers per line */
 } bfvm_x_sweep;
 
 /* Kernel selection and rounding.  persistent_wgs_per_cu: 0 = the default per

@@ -667,11 +667,14 @@ int build_tiles(bfvm_ctx* c)
 			}
 			// rows too long for one lane: two lanes per line (the top and bottom
 			// halves in two waves)
-			else if (axis == 0 && (xs == BFVM_X_CHUNKED || xs == BFVM_X_CHUNKED4) && xgt_bucket(n, c->S) &&
+			else if (axis == 0 &&
+					 (((xs == BFVM_X_CHUNKED || xs == BFVM_X_CHUNKED4 || xs == BFVM_X_TWO_LANE) &&
+					   xgt_bucket(n, c->S)) ||
+					  (xs == BFVM_X_TWO_LANE && xgt_short_bucket(n, c->S))) &&
 					 c->fp64 && c->n[1] * c->n[2] >= xg_rows(c->S) && !(c->dims == 1 && c->face_mask))
 			{
 				c->variant[axis] = kSweepXGT;
-				c->nmax[axis] = xgt_bucket(n, c->S);
+				c->nmax[axis] = xgt_bucket(n, c->S) ? xgt_bucket(n, c->S) : xgt_short_bucket(n, c->S);
 			}
 			// the scan kernel: asked for, or the default's chunked choice did not
 			// take the shape (too few rows for a wave, or 1-D with faces)
@@ -2461,7 +2464,7 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 		return c->fail(BFVM_ERR_INVALID, "persistent_wgs_per_cu must be in [-1, 64]");
 	if (t->force_lds_sweeps != 0 && t->force_lds_sweeps != 1)
 		return c->fail(BFVM_ERR_INVALID, "force_lds_sweeps must be 0 or 1");
-	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_CHUNKED4)
+	if (t->x_sweep < BFVM_X_AUTO || t->x_sweep > BFVM_X_TWO_LANE)
 		return c->fail(BFVM_ERR_INVALID, "x_sweep must be one of the bfvm_x_sweep values");
 	if (t->pipeline != 0 && t->pipeline != 1)
 		return c->fail(BFVM_ERR_INVALID, "pipeline must be 0 or 1");
@@ -2662,6 +2665,7 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	cand.push_back(T(BFVM_X_TRANSPOSE, 0, 0));
 	cand.push_back(T(BFVM_X_PIPELINED, 0, 0));
 	cand.push_back(T(BFVM_X_CHUNKED, 0, 0));
+	cand.push_back(T(BFVM_X_TWO_LANE, 0, 0));
 	cand.push_back(T(BFVM_X_AUTO, 1, 0));
 	cand.push_back(T(BFVM_X_AUTO, 0, 1));
 	cand.push_back(T(BFVM_X_AUTO, 0, 2));

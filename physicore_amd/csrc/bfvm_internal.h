@@ -297,6 +297,9 @@ __host__ __device__ constexpr int xg_rows(int S) { return S % 2 ? (64 / S) & ~1 
 // NH - 31; 0 outside that range.  The factor table carries kXgtPrefix zeros.
 int xgt_bucket(int n, int S);
 constexpr int kXgtPrefix = 32;
+// The same kernel on rows one lane would hold (65..xg_max_n(S) voxels), on
+// request: NH = ceil(n/2) rounded up to 16 (at least 48), 0 outside that range.
+int xgt_short_bucket(int n, int S);
 
 // LDS bytes of a sweep launch of `threads` per workgroup: factor tables
 // (fac_cap doubles each) plus the tile.

@@ -259,6 +259,13 @@ int xgt_bucket(int n, int S)
 	return nh <= xg_max_n(S) ? nh : 0;
 }
 
+int xgt_short_bucket(int n, int S)
+{
+	if (S < 1 || S > 8 || n < 65 || n > xg_max_n(S))
+		return 0;
+	return std::max(48, ((n + 1) / 2 + 15) / 16 * 16);
+}
+
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap, int threads)
 {
 	const size_t waves = (size_t)threads / 64;
@@ -314,10 +321,11 @@ hipError_t init_sweep_kernels()
 				return e;
 		}
 	for (int S = 1; S <= 8; S++)
-		for (int n = xg_max_n(S) + 1; n <= 2 * xg_max_n(S); n++)
+		for (int n = 65; n <= 2 * xg_max_n(S); n++)
 		{
-			const int nh = xgt_bucket(n, S);
-			if (nh == 0 || (n > xg_max_n(S) + 1 && xgt_bucket(n - 1, S) == nh))
+			const int nh = n > xg_max_n(S) ? xgt_bucket(n, S) : xgt_short_bucket(n, S);
+			const int prev = n - 1 > xg_max_n(S) ? xgt_bucket(n - 1, S) : xgt_short_bucket(n - 1, S);
+			if (nh == 0 || prev == nh)
 				continue;
 			sweep::kernel_fn k = sweep::pick(0, kSweepXGT, nh, true, kFinNone, S);
 			if (!k)

@@ -1,5 +1,5 @@
 // Instantiations of the two-lane chunked x sweep for 4 substrates, rows of
-// 201..400 voxels (half-row buckets).
+// 65..400 voxels (half-row buckets).
 #include "bfvm_xgt_kernels.h"
 
 namespace bfvm {
@@ -12,6 +12,10 @@ kernel_fn pick_xgt4(int nmax)
 		return sweep_xgt_kernel<N, 4>;
 	switch (nmax)
 	{
+		BFVM_BUCKET(48)
+		BFVM_BUCKET(64)
+		BFVM_BUCKET(80)
+		BFVM_BUCKET(96)
 		BFVM_BUCKET(112)
 		BFVM_BUCKET(128)
 		BFVM_BUCKET(144)

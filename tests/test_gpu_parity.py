@@ -67,7 +67,8 @@ SHAPES = [
 # pipelined, chunked transpose), pipelined y/z lines, round-to-nearest stores,
 # and every line solved inside an LDS tile
 VARIANTS = {"default": {}, "xstrided": {"x_sweep": 1}, "xrows": {"x_sweep": 2}, "xscan": {"x_sweep": 3},
-            "xpipe": {"x_sweep": 4}, "xchunk": {"x_sweep": 5}, "xchunk4": {"x_sweep": 6}, "pipe": {"pipeline": 1},
+            "xpipe": {"x_sweep": 4}, "xchunk": {"x_sweep": 5}, "xchunk4": {"x_sweep": 6}, "xtwo": {"x_sweep": 7},
+            "pipe": {"pipeline": 1},
             "nearest": {"round_nearest": 1}, "lds": {"force_lds_sweeps": 1}}
 
 
@@ -228,6 +229,30 @@ def test_two_lane_x_sweep_matches_oracle(dims, grid, S):
     want = p.diffuse(f0.astype(np.float32).astype(np.float64))
     d = gpu_domain(p, field=f0)
     d.set_tuning(x_sweep=6 if S in (4, 8) else 5)
+    d.step(1)
+    err = rel_linf(d.field(), want)
+    assert err.max() < 2e-6, err
+
+
+# rows a lane could hold, split over two lanes on request (x_sweep 7): the
+# shortest half-row bucket (48), bucket edges, partial wave pairs, every S
+SHORT_TWO_LANE_SHAPES = [
+    (3, (65, 40, 3), 1), (3, (80, 33, 2), 2), (3, (97, 21, 2), 3), (3, (128, 17, 3), 4),
+    (3, (160, 9, 2), 5), (3, (176, 7, 3), 6), (3, (192, 9, 2), 7), (3, (200, 11, 2), 8),
+    (2, (96, 70, 1), 4), (3, (113, 13, 3), 8), (3, (66, 19, 5), 5), (3, (200, 16, 2), 2),
+]
+
+
+@pytest.mark.parametrize("dims,grid,S", SHORT_TWO_LANE_SHAPES)
+def test_short_rows_on_two_lanes_match_oracle(dims, grid, S):
+    rng = np.random.default_rng(hash((dims, grid, S, 17)) % 2**32)
+    D = 10 ** rng.uniform(0, 5, S)
+    lam = rng.uniform(0.01, 1, S)
+    p = orc.Problem(dims, grid, (20, 20, 20), 0.01, D, lam)
+    f0 = rng.uniform(0, 1, p.shape)
+    want = p.diffuse(f0.astype(np.float32).astype(np.float64))
+    d = gpu_domain(p, field=f0)
+    d.set_tuning(x_sweep=7)
     d.step(1)
     err = rel_linf(d.field(), want)
     assert err.max() < 2e-6, err
