@@ -706,9 +706,10 @@ int build_tiles(bfvm_ctx* c)
 			const bool padded = c->variant[axis] == kSweepReg || c->variant[axis] == kSweepRows ||
 								c->variant[axis] == kSweepPipe || c->variant[axis] == kSweepXT ||
 								c->variant[axis] == kSweepXG;
-			c->fac_cap[axis] = c->variant[axis] == kSweepTwist  ? (c->nmax[axis] + 16) * c->S
-							   : c->variant[axis] == kSweepXGT ? (c->nmax[axis] + kXgtPrefix) * c->S
-															   : (padded ? c->nmax[axis] : n) * c->S;
+			c->fac_cap[axis] = c->variant[axis] == kSweepTwist  ? fac_row(c->nmax[axis] + 16) * c->S
+							   : c->variant[axis] == kSweepXGT ? fac_row(c->nmax[axis] + kXgtPrefix) * c->S
+							   : padded						   ? fac_row(c->nmax[axis]) * c->S
+															   : n * c->S;
 			if (c->variant[axis] == kSweepPipe)
 			{
 				domain_pipe_tiles(axis, 0, (int)c->n[0], (int)c->n[1], (int)c->n[2], c->S, c->pipe[axis], t);
@@ -1237,7 +1238,7 @@ int batch_prepare(bfvm_batch* b)
 					g.buf_floats = std::max(g.buf_floats, c->pipe[axis].buf_floats);
 					g.x_rows = c->pipe[axis].x_rows;
 				}
-				const size_t fac = (size_t)(L->nmax * maxS + 64) * sizeof(double);
+				const size_t fac = (size_t)(fac_row(L->nmax) * maxS + 64) * sizeof(double);
 				const size_t buf = (size_t)g.buf_floats * sizeof(float);
 				const int64_t tiles = (lines + 63) / 64;
 				g.waves = fac + buf > (size_t)kMaxLdsBytes
@@ -1247,7 +1248,7 @@ int batch_prepare(bfvm_batch* b)
 					return b->fail(BFVM_ERR_UNSUPPORTED, "pipelined sweep group does not fit the LDS");
 				g.lds_bytes = fac + (size_t)g.waves * buf;
 				L->pg = g;
-				L->fac_cap = L->nmax * maxS;
+				L->fac_cap = fac_row(L->nmax) * maxS;
 				L->lds = g.lds_bytes;
 				for (int i : doms)
 				{

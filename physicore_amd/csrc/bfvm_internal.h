@@ -301,6 +301,14 @@ constexpr int kXgtPrefix = 32;
 // request: NH = ceil(n/2) rounded up to 16 (at least 48), 0 outside that range.
 int xgt_short_bucket(int n, int S);
 
+// Row stride (doubles) of the factor tables the one-lane register kernels
+// keep in LDS, [s][fac_row(NMAX)]: the bucket plus 2.  A wave's lanes read the
+// rows of up to S substrates at one step; with a stride that is a multiple of
+// 32 doubles (buckets of 32, 64, ..., 160, 192) every row starts in the same
+// LDS bank and the reads conflict S ways, while NMAX + 2 (NMAX a multiple of
+// 8) puts 8 rows in distinct banks and keeps 16-byte alignment.
+__host__ __device__ constexpr int fac_row(int nmax) { return nmax + 2; }
+
 // LDS bytes of a sweep launch of `threads` per workgroup: factor tables
 // (fac_cap doubles each) plus the tile.
 size_t sweep_lds_bytes(int axis, int variant, int n, int S, int L, int fac_cap, int threads = kSweepThreads);

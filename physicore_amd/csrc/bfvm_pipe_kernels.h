@@ -154,9 +154,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 			__syncthreads();
 			const int n = AXIS == 0 ? d.nx : (AXIS == 1 ? d.ny : d.nz);
 			const double* gb = d.fb[AXIS];
-			for (int k = threadIdx.x; k < NMAX * d.S; k += blockDim.x)
+			for (int k = threadIdx.x; k < fac_row(NMAX) * d.S; k += blockDim.x)
 			{
-				const int s = k / NMAX, i = k - s * NMAX;
+				const int s = k / fac_row(NMAX), i = k - s * fac_row(NMAX);
 				fb[k] = i < n ? gb[i * d.S + s] : 0.0;
 			}
 			for (int k = threadIdx.x; k < d.S; k += blockDim.x)
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_pipe_kernel(const DomainD
 		const LineKey kf = line_key(d, 2 * AXIS, line);
 		const LineKey ko = line_key(d, 2 * AXIS + 1, line);
 
-		const double* lb = fb + cs * NMAX;
+		const double* lb = fb + cs * fac_row(NMAX);
 		const double cc = fc[cs];
 		int sn = n - 1, sstride4 = stride4;
 		asm volatile("" : "+s"(sn), "+s"(sstride4));

@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 		if (tr.dom != cur)
 		{
 			__syncthreads();
-			load_factors(d, AXIS, n, NMAX, fb, fe, fc);
+			load_factors(d, AXIS, n, fac_row(NMAX), fb, fe, fc);
 			__syncthreads();
 			cur = tr.dom;
 		}
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 					return __builtin_bit_cast(float,
 											  (__builtin_bit_cast(unsigned, r) & m) | (__builtin_bit_cast(unsigned, x) & ~m));
 				};
-				thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+				thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * fac_row(NMAX), fe + cs * fac_row(NMAX), fc[cs], kf, ko,
 												[&](int i, float val) {
 													buf_store(rsrc, voff, min(i, sn) * sstride4, bulk(i, lohi ? fv : val));
 												});
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 			}
 			else
 			{
-				thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+				thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * fac_row(NMAX), fe + cs * fac_row(NMAX), fc[cs], kf, ko,
 												[&](int i, float val) {
 													buf_store(rsrc, voff, min(i, sn) * sstride4, lohi ? fv : val);
 												});
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_cols_kernel(const DomainD
 		}
 		else
 		{
-			thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * NMAX, fe + cs * NMAX, fc[cs], kf, ko,
+			thomas_regs_streamed<NMAX, ACC>(v, ld, fb + cs * fac_row(NMAX), fe + cs * fac_row(NMAX), fc[cs], kf, ko,
 											[&](int i, float val) { buf_store(rsrc, voff, min(i, sn) * sstride4, val); });
 		}
 	}
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_rows_kernel(const DomainD
 		const int n = d.nx, S = d.S, L = d.L;
 		if (tr.dom != cur)
 		{
-			load_factors(d, 0, n, NMAX, fb, fe, fc); // the loop-end barrier of the previous tile orders this
+			load_factors(d, 0, n, fac_row(NMAX), fb, fe, fc); // the loop-end barrier of the previous tile orders this
 			cur = tr.dom;
 		}
 		const int rows = tr.b;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_rows_kernel(const DomainD
 			int sn = n - 1, sS = S;
 			asm volatile("" : "+s"(sn), "+s"(sS));
 			const uint32_t f = (uint32_t)((tr.a + r) * S + s); // line (row, s)
-			thomas_regs<NMAX, ACC>(v, fb + s * NMAX, fe + s * NMAX, fc[s], line_key(d, kRoundFwdX, f),
+			thomas_regs<NMAX, ACC>(v, fb + s * fac_row(NMAX), fe + s * fac_row(NMAX), fc[s], line_key(d, kRoundFwdX, f),
 								   line_key(d, kRoundOutX, f), [&](int i, float val) { line[min(i, sn) * sS] = val; });
 		}
 		__syncthreads();

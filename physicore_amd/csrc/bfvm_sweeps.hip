@@ -400,7 +400,7 @@ PipeGeometry pipe_geometry(int axis, int n, int nmax, int S, int L, int64_t line
 	g.S = S;
 	if (nmax <= 0 || L % 4 != 0 || S > 64 || lines_in_tiles <= 0 || (axis == 0 && !sweep::pipe_x_supports(S)))
 		return g;
-	const size_t fac = (size_t)(nmax * S + 64) * sizeof(double);
+	const size_t fac = (size_t)(fac_row(nmax) * S + 64) * sizeof(double);
 	if (axis == 0)
 	{
 		g.x_rows = 64 / S;

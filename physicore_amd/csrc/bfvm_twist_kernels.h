@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_twist_kernel(const Domain
 {
 	static_assert(AXIS == 1 || AXIS == 2, "twisted lines are y or z lines");
 	static_assert(NH > kTwistPrefix && NH <= kRegMaxN, "half-line bucket");
-	constexpr int W = NH + kTwistPrefix; // table row per substrate
+	constexpr int W = fac_row(NH + kTwistPrefix); // table row per substrate (padded: see fac_row)
 	extern __shared__ double lds[];
 	double* fb = lds;				// [s][W]: zeros, then b'_k (the second table of the
 									// layout is unused: e_k = c b'_k is formed in registers,

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kSweepThreads, 2) void sweep_xgt_kernel(const Domai
 	constexpr int RP = 64 / xg_gcd(CW, 64);
 	constexpr int NQ = R * CW / 64;
 	constexpr int NCH = (NH + kXgChunk - 1) / kXgChunk;
-	constexpr int W = NH + kXgtPrefix; // factor table row per substrate
+	constexpr int W = fac_row(NH + kXgtPrefix); // factor table row per substrate (padded: see fac_row)
 	static_assert(NQ % P == 0 && R % RP == 0, "whole periods per chunk");
 	// chunk k: voxels of the half, and whether its loads/stores take the
 	// per-lane path (a partial chunk, or one that can reach the bottom half's

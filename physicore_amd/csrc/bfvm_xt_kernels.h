@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xt_kernel(const DomainDev
 		if (tr.dom != cur)
 		{
 			__syncthreads();
-			load_factors(d, 0, n, NMAX, fb, fe, fc);
+			load_factors(d, 0, n, fac_row(NMAX), fb, fe, fc);
 			__syncthreads();
 			cur = tr.dom;
 		}
@@ -135,8 +135,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_xt_kernel(const DomainDev
 		const int line = (row0 + rl) * S + sl;
 		const LineKey kf = line_key(d, kRoundFwdX, (uint32_t)line);
 		const LineKey ko = line_key(d, kRoundOutX, (uint32_t)line);
-		const double* lb = fb + sl * NMAX;
-		const double* le = fe + sl * NMAX;
+		const double* lb = fb + sl * fac_row(NMAX);
+		const double* le = fe + sl * fac_row(NMAX);
 		const double cc = fc[sl];
 		float v[NMAX];
 		static_for<NCH>([&](auto K) {
