@@ -288,8 +288,16 @@ typedef enum bfvm_x_sweep
  * solved.  round_nearest = 1 rounds FP64 results to the FP32 field to nearest
  * instead of stochastically (see bfvm_precision).  lines_per_block: lines per
  * workgroup of the one-lane register kernels (y/z column lines, strided and
- * chunked x lines): 64, 128 or 256 (one, two or four waves; 0 = 256).  The
- * results do not depend on it. */
+ * chunked x lines): 64, 128 or 256 (one, two or four waves; 0 = 256).
+ * split_xy (3-D domains whose x sweep runs register or scan lines and whose
+ * y sweep runs column lines): 1 = launch the x sweep in two halves on two
+ * streams, the y lines of the planes the first half covers starting as soon
+ * as it is done (the x sweep's last partial round of waves then overlaps y
+ * work instead of running alone); 2 = one launch per axis; 0 = the default:
+ * halves when the x sweep needs more than one and at most three rounds of
+ * the GPU's register file and the y lines hold two waves per SIMD (buckets
+ * up to 208, or two lanes per line).  Neither setting changes the results.
+ * bfvm_time_stages always times one launch per axis. */
 typedef struct bfvm_tuning
 {
 	int32_t persistent_wgs_per_cu;
@@ -298,7 +306,8 @@ typedef struct bfvm_tuning
 	int32_t pipeline;
 	int32_t round_nearest;
 	int32_t lines_per_block;
-	int32_t reserved[2];
+	int32_t split_xy;
+	int32_t reserved[1];
 } bfvm_tuning;
 int bfvm_get_tuning(const bfvm_ctx* ctx, bfvm_tuning* out);
 int bfvm_set_tuning(bfvm_ctx* ctx, const bfvm_tuning* t);

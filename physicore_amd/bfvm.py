@@ -43,7 +43,8 @@ class DomainDesc(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("persistent_wgs_per_cu", C.c_int32), ("force_lds_sweeps", C.c_int32),
                 ("x_sweep", C.c_int32), ("pipeline", C.c_int32), ("round_nearest", C.c_int32),
-                ("lines_per_block", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("lines_per_block", C.c_int32), ("split_xy", C.c_int32),
+                ("reserved", C.c_int32 * 1)]
 
 
 _LIB = None
@@ -343,10 +344,10 @@ class Domain:
         return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu,
                 "force_lds_sweeps": t.force_lds_sweeps, "x_sweep": t.x_sweep,
                 "pipeline": t.pipeline, "round_nearest": t.round_nearest,
-                "lines_per_block": t.lines_per_block}
+                "lines_per_block": t.lines_per_block, "split_xy": t.split_xy}
 
     def set_tuning(self, persistent_wgs_per_cu=0, force_lds_sweeps=0, x_sweep=0, pipeline=0,
-                   round_nearest=0, lines_per_block=0):
+                   round_nearest=0, lines_per_block=0, split_xy=0):
         t = Tuning()
         t.persistent_wgs_per_cu = int(persistent_wgs_per_cu)
         t.force_lds_sweeps = int(force_lds_sweeps)
@@ -354,13 +355,14 @@ class Domain:
         t.pipeline = int(pipeline)
         t.round_nearest = int(round_nearest)
         t.lines_per_block = int(lines_per_block)
+        t.split_xy = int(split_xy)
         self._check(lib().bfvm_set_tuning(self._h, C.byref(t)))
 
 
 def _tuning_dict(t):
     return {"persistent_wgs_per_cu": t.persistent_wgs_per_cu, "force_lds_sweeps": t.force_lds_sweeps,
             "x_sweep": t.x_sweep, "pipeline": t.pipeline, "round_nearest": t.round_nearest,
-            "lines_per_block": t.lines_per_block}
+            "lines_per_block": t.lines_per_block, "split_xy": t.split_xy}
 
 
 class Batch:

@@ -385,6 +385,13 @@ struct bfvm_ctx
 
 	dvec<TileRef> tiles[3];
 	int ntiles[3] = { 0, 0, 0 };
+	// the x sweep in two halves, the second on `side`; the y tiles of the
+	// planes the first half covers whole follow it on the context stream, the
+	// others follow the second half (tile counts of the first halves; 0: one
+	// launch per axis)
+	int split_x = 0, split_y = 0;
+	hipStream_t side = nullptr;
+	hipEvent_t ev_split[3] = {};
 	int fac_cap[3] = { 0, 0, 0 };
 	int variant[3] = { 0, 0, 0 };
 	int nmax[3] = { 0, 0, 0 };
@@ -446,6 +453,13 @@ struct bfvm_ctx
 				(void)hipEventDestroy(e);
 		if (snap_taken)
 			(void)hipEventDestroy(snap_taken);
+		if (side)
+			(void)hipStreamSynchronize(side);
+		for (hipEvent_t e : ev_split)
+			if (e)
+				(void)hipEventDestroy(e);
+		if (side)
+			(void)hipStreamDestroy(side);
 		if (copy_stream)
 			(void)hipStreamDestroy(copy_stream);
 		if (stream)
@@ -620,6 +634,7 @@ void domain_tiles(int axis, int variant, int dom, int nx, int ny, int nz, int S,
 
 int build_tiles(bfvm_ctx* c)
 {
+	std::vector<TileRef> host_tiles[3];
 	for (int axis = 0; axis < 3; axis++)
 	{
 		std::vector<TileRef> t;
@@ -730,6 +745,51 @@ int build_tiles(bfvm_ctx* c)
 		}
 		c->ntiles[axis] = (int)t.size();
 		TRY_HIP(c, c->tiles[axis].upload(t.data(), t.size(), c->stream));
+		host_tiles[axis] = std::move(t);
+	}
+	// x in two halves (3-D domains; x tiles of rows or of lines, y tiles of
+	// columns): the last round of x waves then overlaps the y tiles of the
+	// first half's planes instead of running alone.  By default only where
+	// that was measured to pay (200^3 x 4 steps 3% faster, 400x400x100 x 2 7%):
+	// the x lanes fill more than one and at most three rounds of the register
+	// file at two waves per SIMD, and the y lines run two waves per SIMD too.
+	// Below one round the halves only add launches (a 176^3 x 4 step is 10%
+	// slower split); y lines of 209..256 voxels (one wave per SIMD) crowd out
+	// the second half (200x245x200 x 4: 6% slower)
+	c->split_x = c->split_y = 0;
+	const int vx = c->variant[0], vy = c->variant[1];
+	bool want = c->tuning.split_xy == 1;
+	if (c->tuning.split_xy == 0)
+	{
+		const int64_t lanes = c->n[1] * c->n[2] * c->S * (vx == kSweepXGT ? 2 : 1);
+		const int64_t round = (int64_t)c->num_cu * 4 * 2 * 64;
+		want = lanes > round && lanes <= 3 * round && vx != kSweepScan &&
+			   (vy == kSweepTwist || (vy == kSweepReg && c->nmax[1] <= 208));
+	}
+	if (want && c->dims == 3 && host_tiles[0].size() >= 2 &&
+		(vx == kSweepXG || vx == kSweepXGT || vx == kSweepXT || vx == kSweepReg || vx == kSweepScan) &&
+		(vy == kSweepReg || vy == kSweepTwist))
+	{
+		const std::vector<TileRef>& tx = host_tiles[0];
+		const std::vector<TileRef>& ty = host_tiles[1];
+		const int kx = (int)tx.size() / 2;
+		// lines (row*S + s) of the first half: those below tile kx's first
+		const int64_t first = vx == kSweepXT || vx == kSweepReg ? (int64_t)tx[kx].a : (int64_t)tx[kx].a * c->S;
+		const int64_t planes = first / c->S / c->n[1]; // planes of whole rows in the first half
+		const int per = vy == kSweepTwist ? kTwistColsPerTile : c->threads[1];
+		int ky = 0;
+		while (ky < (int)ty.size() && (int64_t)ty[ky].a + per <= planes * c->L)
+			ky++;
+		if (ky > 0 && ky < (int)ty.size())
+		{
+			if (!c->side)
+				TRY_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+			for (hipEvent_t& e : c->ev_split)
+				if (!e)
+					TRY_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+			c->split_x = kx;
+			c->split_y = ky;
+		}
 	}
 	return BFVM_OK;
 }
@@ -767,6 +827,18 @@ hipError_t launch_axis_sweep(bfvm_ctx* c, int axis, const DomainDev* dd)
 						c->threads[axis]);
 }
 
+// Tiles [first, first + count) of one axis' sweep on `stream` (an axis that
+// is not the last: no face or bulk stage).
+hipError_t launch_axis_part(bfvm_ctx* c, int axis, const DomainDev* dd, int first, int count, hipStream_t stream)
+{
+	const int per_cu = c->tuning.persistent_wgs_per_cu > 0 ? c->tuning.persistent_wgs_per_cu
+					   : c->variant[axis] == kSweepScan	   ? 2
+														   : 0;
+	const int grid = per_cu > 0 ? std::min(count, c->num_cu * per_cu) : count;
+	return launch_sweep(axis, c->variant[axis], c->nmax[axis], c->fp64, kFinNone, c->S, dd, c->tiles[axis].p + first,
+						count, 1, c->fac_cap[axis], c->lds[axis], grid, stream, c->threads[axis]);
+}
+
 // One step on the stream: sweeps x, y, z (faces fused into the last), interior
 // Dirichlet, bulk, cells.  Stage order of openmp_solver.cpp:27-37.  `ev` (7
 // events or null) brackets the stages for timing.  `dd` is the domain table
@@ -778,12 +850,30 @@ int enqueue_step(bfvm_ctx* c, hipEvent_t* ev, const DomainDev* dd = nullptr, uin
 		dd = c->ddom.p;
 	auto mark = [&](int k) -> hipError_t { return ev ? hipEventRecord(ev[k], c->stream) : hipSuccess; };
 	TRY_HIP(c, mark(0));
-	for (int axis = 0; axis < 3; axis++)
+	if (c->split_x && !ev)
 	{
-		if (axis < c->dims)
-			TRY_HIP(c, launch_axis_sweep(c, axis, dd));
-		TRY_HIP(c, mark(axis + 1));
+		// x halves A (context stream) and B (side stream); y of A's planes
+		// after A, the other y tiles after both; z after all of y
+		const int nx = c->ntiles[0] - c->split_x, ny = c->ntiles[1] - c->split_y;
+		TRY_HIP(c, hipEventRecord(c->ev_split[0], c->stream));
+		TRY_HIP(c, hipStreamWaitEvent(c->side, c->ev_split[0], 0));
+		TRY_HIP(c, launch_axis_part(c, 0, dd, 0, c->split_x, c->stream));
+		TRY_HIP(c, hipEventRecord(c->ev_split[1], c->stream));
+		TRY_HIP(c, launch_axis_part(c, 0, dd, c->split_x, nx, c->side));
+		TRY_HIP(c, launch_axis_part(c, 1, dd, 0, c->split_y, c->stream));
+		TRY_HIP(c, hipStreamWaitEvent(c->side, c->ev_split[1], 0));
+		TRY_HIP(c, launch_axis_part(c, 1, dd, c->split_y, ny, c->side));
+		TRY_HIP(c, hipEventRecord(c->ev_split[2], c->side));
+		TRY_HIP(c, hipStreamWaitEvent(c->stream, c->ev_split[2], 0));
+		TRY_HIP(c, launch_axis_sweep(c, 2, dd));
 	}
+	else
+		for (int axis = 0; axis < 3; axis++)
+		{
+			if (axis < c->dims)
+				TRY_HIP(c, launch_axis_sweep(c, axis, dd));
+			TRY_HIP(c, mark(axis + 1));
+		}
 	if (c->dir_count)
 		TRY_HIP(c, launch_dirichlet_interior(dd, c->dir_refs.p, c->dir_count, c->stream));
 	TRY_HIP(c, mark(4));
@@ -2474,6 +2564,8 @@ int bfvm_set_tuning(bfvm_ctx* c, const bfvm_tuning* t)
 	if (t->lines_per_block != 0 && t->lines_per_block != 64 && t->lines_per_block != 128 &&
 		t->lines_per_block != 256)
 		return c->fail(BFVM_ERR_INVALID, "lines_per_block must be 0, 64, 128 or 256");
+	if (t->split_xy < 0 || t->split_xy > 2)
+		return c->fail(BFVM_ERR_INVALID, "split_xy must be 0, 1 or 2");
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
 	c->tuning = *t;
 	// the kernel variants decide whether a uniform bulk is applied by the last
@@ -2679,6 +2771,7 @@ int bfvm_batch_autotune(bfvm_batch* b, int64_t steps, bfvm_tuning* chosen)
 	auto apply = [&](bfvm_ctx* c, const bfvm_tuning& t0) {
 		bfvm_tuning t = t0;
 		t.round_nearest = c->tuning.round_nearest;
+		t.split_xy = c->tuning.split_xy;
 		return bfvm_set_tuning(c, &t);
 	};
 	// the candidates step the domains' own buffers: what a step changes (the

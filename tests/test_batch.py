@@ -180,7 +180,7 @@ def test_batch_autotune_leaves_domains_as_they_were(tmp_path):
     cache = str(tmp_path / "tuning.json")
     t = b.autotune(steps=2, cache=cache)
     assert set(t) == {"persistent_wgs_per_cu", "force_lds_sweeps", "x_sweep", "pipeline", "round_nearest",
-                      "lines_per_block"}
+                      "lines_per_block", "split_xy"}
     for k, (d, f) in enumerate(zip(doms, before)):
         assert np.array_equal(d.field(), f), k
     b.step(3)

@@ -204,6 +204,56 @@ def test_lines_per_block_in_a_batch():
         assert np.array_equal(a.field(), m.field())
 
 
+# the x sweep in two halves on two streams (split_xy 1) against one launch per
+# axis (split_xy 2); each shape splits both sweeps inside the domain: x kernels
+# chunked 4-byte (S 4, rows of 200), chunked 16-byte (S 8), two lanes per line
+# (rows of 260), strided lines and the scan; y column lines and two lanes per
+# line (y lines of 300)
+SPLIT_SHAPES = [((200, 13, 30), 4, 0), ((120, 20, 24), 8, 0), ((260, 16, 20), 4, 0), ((40, 300, 12), 2, 0),
+                ((200, 13, 30), 4, 1), ((300, 10, 12), 1, 3), ((61, 37, 9), 3, 0)]
+
+
+@pytest.mark.parametrize("grid,S,x_sweep", SPLIT_SHAPES)
+def test_split_xy_same_results(grid, S, x_sweep):
+    """The halves change only the launches: the fields equal one launch per
+    axis bit for bit over eager steps and replayed graphs"""
+    rng = np.random.default_rng(hash((grid, S, 65)) % 2**32)
+    p = orc.Problem(3, grid, (20, 20, 20), 0.01, 10 ** rng.uniform(0, 5, S), rng.uniform(0.01, 1, S))
+    f0 = rng.uniform(0, 1, p.shape)
+    out = []
+    for split in (1, 2):
+        d = gpu_domain(p, field=f0)
+        d.set_tuning(x_sweep=x_sweep, split_xy=split)
+        d.step(3)
+        d.step_graph(20, 10)
+        out.append(d.field())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_split_xy_full_step_workload():
+    """A whole step (Dirichlet face, bulk sphere, cells) with the halves equals
+    one launch per axis bit for bit, and the default (halves where they pay)
+    equals both"""
+    from physicore_amd import workloads as wl
+    w = wl.c2(side=64, n_cells=5000)
+    out = []
+    for split in (1, 2, 0):
+        d = wl.make_domain(w)
+        d.set_tuning(split_xy=split)
+        d.step(2)
+        d.step_graph(10, 10)
+        out.append(d.field())
+        d.close()
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_split_xy_rejects_other_values():
+    from physicore_amd import bfvm
+    d = gpu_domain(orc.Problem(3, (8, 8, 8), (20, 20, 20), 0.01, [1e3], [0.1]))
+    with pytest.raises(bfvm.BfvmError):
+        d.set_tuning(split_xy=3)
+
+
 # rows longer than one lane holds (past 176/192/200 voxels for S = 1/5,7/others):
 # the two-lane chunked kernel, its bucket edges, partial wave pairs, and the
 # longest rows of each S
