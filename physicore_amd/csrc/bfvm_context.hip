@@ -772,7 +772,10 @@ int build_tiles(bfvm_ctx* c)
 	{
 		const std::vector<TileRef>& tx = host_tiles[0];
 		const std::vector<TileRef>& ty = host_tiles[1];
-		const int kx = (int)tx.size() / 2;
+		// the first half takes 60% of the x tiles (measured against 50-80%:
+		// 200^3 x 4 0.150 against 0.152 ms/step at 50%, 400x400x100 x 2
+		// 0.145 against 0.148)
+		const int kx = std::clamp((int)(tx.size() * 3 / 5), 1, (int)tx.size() - 1);
 		// lines (row*S + s) of the first half: those below tile kx's first
 		const int64_t first = vx == kSweepXT || vx == kSweepReg ? (int64_t)tx[kx].a : (int64_t)tx[kx].a * c->S;
 		const int64_t planes = first / c->S / c->n[1]; // planes of whole rows in the first half
