@@ -283,10 +283,18 @@ def bench_plain(args):
 
     # in-step launch durations: 50 steps queued back to back on the solver
     # stream with HIP events between the launches (each figure includes the
-    # gap before its launch); the roofline is priced on these.  Then, as a
+    # gap before its launch; these steps launch each sweep once, while the
+    # timed steps above run the x sweep in two halves on two streams, the
+    # second overlapping the y lines of the first half's planes); the
+    # roofline is priced on these.  Then, as a
     # kernel-level figure, each sweep's average over 50 launches of that sweep
     # alone between two events (the extra sweeps only change the field after
     # the timed region)
+    plan = d.sweep_plan()
+    schedule = dict(plan, note=(
+        "x sweep in two halves on two streams (split_x and x_tiles - split_x tiles); the y tiles of the planes "
+        "the first half covers (split_y) start when it is done, the others after both halves; z after all of "
+        "y; results bit-identical to one launch per sweep" if plan["split_x"] else "one launch per sweep"))
     stages = d.time_stages(min(args.steps, 50))
     sweeps = {a: d.time_sweep(i, 50) for i, a in enumerate("xyz")}
     sweep_bytes = units * 8  # one sweep reads and writes every FP32 value once
@@ -324,10 +332,12 @@ def bench_plain(args):
                      "traffic_source": traffic,
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "avg_launch_us": stages[dominant],
-                     "timing": "in-step: HIP events between the launches of 50 back-to-back steps",
+                     "timing": "in-step: HIP events between the launches of 50 back-to-back steps "
+                               "with one launch per sweep",
                      "kernel_level": {"avg_launch_us": sweeps[dominant],
                                       "achieved": sweep_bytes / (sweeps[dominant] * 1e-6) / 1e9,
                                       "timing": "50 launches of the sweep alone between two events"}},
+        "schedule": schedule,
         "step_roofline": {"bytes_per_step": units * 24, "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS,
                           "diffusion_us": diff_us,

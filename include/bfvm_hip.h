@@ -242,6 +242,17 @@ int bfvm_time_stages(bfvm_ctx* ctx, int64_t steps, double* out_us);
  * that axis (no counterpart in the reference). */
 int bfvm_time_sweep(bfvm_ctx* ctx, int32_t axis, int64_t launches, double* out_us);
 
+/* The sweeps a step launches (diagnostics; no counterpart in the reference).
+ * kernels[axis] (3 entries; axes past dims are 0): 0 = lines solved inside
+ * LDS tiles, 1 = one line per lane in VGPRs read from HBM (y/z columns,
+ * strided x lines), 2 = x scan, 3 = x rows staged in LDS, 4 = pipelined
+ * register lines, 5 = two lanes per y/z line, 6 = x chunked 16-byte
+ * transposes, 7 = x chunked 4-byte transposes, 8 = x two lanes per line.
+ * split (4 entries, may be NULL): the tiles of the first halves of the x and
+ * y sweeps when the x sweep runs in two halves (bfvm_tuning.split_xy; 0 and
+ * 0 when it does not), then the tile counts of the whole x and y sweeps. */
+int bfvm_sweep_plan(const bfvm_ctx* ctx, int32_t* kernels, int32_t* split);
+
 /* Kernel of the x sweep.  BFVM_X_AUTO picks per shape (measured on MI355X):
  * the chunked kernels for S = 4, 8 on rows of 96..200 voxels (16-byte chunks
  * up to 176 voxels, 4-byte chunks above; S = 4 also up to 32), for S = 3 and

@@ -81,6 +81,7 @@ _SIGS = {
     "bfvm_time_stages": ([C.c_void_p, C.c_int64, f64p], C.c_int),
     "bfvm_time_sweep": ([C.c_void_p, C.c_int32, C.c_int64, f64p], C.c_int),
     "bfvm_get_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
+    "bfvm_sweep_plan": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "bfvm_set_tuning": ([C.c_void_p, C.POINTER(Tuning)], C.c_int),
     "bfvm_batch_create": ([C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
     "bfvm_batch_destroy": ([C.c_void_p], None),
@@ -337,6 +338,16 @@ class Domain:
         out = np.zeros(1)
         self._check(lib().bfvm_time_sweep(self._h, int(axis), int(launches), _p(out)))
         return float(out[0])
+
+    def sweep_plan(self):
+        """The kernel family of each axis' sweep and, when the x sweep runs
+        in two halves, the tiles of the first halves of x and y."""
+        k = np.zeros(3, dtype=np.int32)
+        sp = np.zeros(4, dtype=np.int32)
+        i32 = C.POINTER(C.c_int32)
+        self._check(lib().bfvm_sweep_plan(self._h, _p(k, i32), _p(sp, i32)))
+        return {"kernels": k.tolist(), "split_x": int(sp[0]), "split_y": int(sp[1]),
+                "x_tiles": int(sp[2]), "y_tiles": int(sp[3])}
 
     def tuning(self):
         t = Tuning()

@@ -2542,6 +2542,22 @@ int bfvm_time_stages(bfvm_ctx* c, int64_t steps, double* out_us)
 	return BFVM_OK;
 }
 
+int bfvm_sweep_plan(const bfvm_ctx* c, int32_t* kernels, int32_t* split)
+{
+	if (!c || !kernels)
+		return BFVM_ERR_INVALID;
+	for (int axis = 0; axis < 3; axis++)
+		kernels[axis] = axis < c->dims ? c->variant[axis] : 0;
+	if (split)
+	{
+		split[0] = c->split_x;
+		split[1] = c->split_y;
+		split[2] = c->ntiles[0];
+		split[3] = c->dims > 1 ? c->ntiles[1] : 0;
+	}
+	return BFVM_OK;
+}
+
 int bfvm_get_tuning(const bfvm_ctx* c, bfvm_tuning* out)
 {
 	if (!c || !out)

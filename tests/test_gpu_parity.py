@@ -224,10 +224,31 @@ def test_split_xy_same_results(grid, S, x_sweep):
     for split in (1, 2):
         d = gpu_domain(p, field=f0)
         d.set_tuning(x_sweep=x_sweep, split_xy=split)
+        plan = d.sweep_plan()
+        if split == 1:
+            assert 0 < plan["split_x"] < plan["x_tiles"] and 0 < plan["split_y"] < plan["y_tiles"], plan
+        else:
+            assert plan["split_x"] == plan["split_y"] == 0
         d.step(3)
         d.step_graph(20, 10)
         out.append(d.field())
     assert np.array_equal(out[0], out[1])
+
+
+# the default: halves when the x lanes fill more than one and at most three
+# rounds of the register file and the y lines run two waves per SIMD
+DEFAULT_SPLIT = [((200, 200, 200), 4, True), ((176, 176, 176), 4, False), ((200, 245, 200), 4, False),
+                 ((400, 400, 100), 2, True), ((250, 250, 250), 4, False), ((160, 160, 160), 8, True),
+                 ((128, 128, 128), 8, False)]
+
+
+@pytest.mark.parametrize("grid,S,split", DEFAULT_SPLIT)
+def test_split_xy_default_choice(grid, S, split):
+    from physicore_amd import bfvm
+    d = bfvm.Domain(3, grid, (20, 20, 20), 0.01, np.full(S, 1e4), np.full(S, 0.1), initial=np.ones(S))
+    plan = d.sweep_plan()
+    d.close()
+    assert (plan["split_x"] > 0) == split, plan
 
 
 def test_split_xy_full_step_workload():
