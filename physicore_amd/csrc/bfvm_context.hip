@@ -758,24 +758,27 @@ int build_tiles(bfvm_ctx* c)
 	// the second half (200x245x200 x 4: 6% slower)
 	c->split_x = c->split_y = 0;
 	const int vx = c->variant[0], vy = c->variant[1];
+	const int64_t lanes = c->n[1] * c->n[2] * c->S * (vx == kSweepXGT ? 2 : 1);
+	const int64_t round = (int64_t)c->num_cu * 4 * 2 * 64;
 	bool want = c->tuning.split_xy == 1;
 	if (c->tuning.split_xy == 0)
-	{
-		const int64_t lanes = c->n[1] * c->n[2] * c->S * (vx == kSweepXGT ? 2 : 1);
-		const int64_t round = (int64_t)c->num_cu * 4 * 2 * 64;
 		want = lanes > round && lanes <= 3 * round && vx != kSweepScan &&
 			   (vy == kSweepTwist || (vy == kSweepReg && c->nmax[1] <= 208));
-	}
 	if (want && c->dims == 3 && host_tiles[0].size() >= 2 &&
 		(vx == kSweepXG || vx == kSweepXGT || vx == kSweepXT || vx == kSweepReg || vx == kSweepScan) &&
 		(vy == kSweepReg || vy == kSweepTwist))
 	{
 		const std::vector<TileRef>& tx = host_tiles[0];
 		const std::vector<TileRef>& ty = host_tiles[1];
-		// the first half takes 60% of the x tiles (measured against 50-80%:
-		// 200^3 x 4 0.150 against 0.152 ms/step at 50%, 400x400x100 x 2
-		// 0.145 against 0.148)
-		const int kx = std::clamp((int)(tx.size() * 3 / 5), 1, (int)tx.size() - 1);
+		// the first half's share of the x tiles, by the rounds the x lanes
+		// need (measured at 30-75%, ms/step: 200x164x200 x 4, 1.0 rounds,
+		// 0.1235 at 70% against 0.1246 at 60%; 200^3 x 4, 1.22 rounds, 0.138
+		// at 60% against 0.141 at 50% and 70%; 192^3 x 6, 1.69 rounds, 0.175
+		// at 50% against 0.184 at 60%; 200^3 x 8, 2.44 rounds, 0.246 at 50%
+		// against 0.251 at 60%)
+		const double r = (double)lanes / (double)round;
+		const int tenths = r < 1.1 ? 7 : r < 1.4 ? 6 : 5;
+		const int kx = std::clamp((int)(tx.size() * tenths / 10), 1, (int)tx.size() - 1);
 		// lines (row*S + s) of the first half: those below tile kx's first
 		const int64_t first = vx == kSweepXT || vx == kSweepReg ? (int64_t)tx[kx].a : (int64_t)tx[kx].a * c->S;
 		const int64_t planes = first / c->S / c->n[1]; // planes of whole rows in the first half
