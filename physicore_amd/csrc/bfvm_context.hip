@@ -846,8 +846,11 @@ hipError_t launch_axis_part(bfvm_ctx* c, int axis, const DomainDev* dd, int firs
 }
 
 // One step on the stream: sweeps x, y, z (faces fused into the last), interior
-// Dirichlet, bulk, cells.  Stage order of openmp_solver.cpp:27-37.  `ev` (7
-// events or null) brackets the stages for timing.  `dd` is the domain table
+// Dirichlet, bulk, cells.  Stage order of openmp_solver.cpp:27-37.  With
+// split_x the x sweep and the y sweep run as halves forked onto the side
+// stream and joined back before z, so the step still starts and ends on the
+// context stream.  `ev` (7 events or null) brackets the stages for timing
+// (one launch per sweep then).  `dd` is the domain table
 // of the step (its step_offset places it in a captured run of steps), and
 // `tick` the steps the step count advances by at its end (0: none).
 int enqueue_step(bfvm_ctx* c, hipEvent_t* ev, const DomainDev* dd = nullptr, uint32_t tick = 1)
