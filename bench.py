@@ -232,6 +232,32 @@ def bench_served(count, rate, seed=46, cpu=True, **server_kw):
     return out
 
 
+def bench_dense(steps=200, cpu=True, budget_s=12.0):
+    """The dense-agent workload (workloads.c4: 400x400x100 x 2 substrates,
+    2e6 cells in a Gaussian spheroid, net export), stepped in 10-step graphs
+    with the atomic and with the sorted (bit-reproducible) cell reduction."""
+    w = wl.c4()
+    out = {"workload": w.name, "steps": steps, "voxel_substrates": w.units, "unit": "voxel*substrate updates/s"}
+    for reduce in ("atomic", "sorted"):
+        d = wl.make_domain(w, reduce=reduce)
+        d.step_graph(20, 10)
+        d.sync()
+        t0 = time.perf_counter()
+        d.step_graph(steps, 10)
+        d.sync()
+        el = time.perf_counter() - t0
+        out[reduce] = {"value": w.units * steps / el, "ms_per_step": el * 1e3 / steps}
+        if reduce == "atomic":
+            out["schedule"] = d.sweep_plan()
+            out["stages_us"] = d.time_stages(30)
+        d.close()
+    out["value"] = out["atomic"]["value"]
+    if cpu:
+        print("bench.py: CPU baseline of the dense-agent workload", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, budget_s)
+    return out
+
+
 def bench_reference(steps=100, cpu=True, budget_s=12.0):
     """The reference's own benchmark workload (openmp_solver/examples/benchmark.cpp:
     250^3 x 4, bulk in every voxel, six Dirichlet faces, 2e6+1 lattice agents,
@@ -366,7 +392,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=8.0)
     ap.add_argument("--only-plain", action="store_true", help="same as --legs plain")
     ap.add_argument("--legs", default="plain,batched,served,reference",
-                    help="comma-separated parts to run: plain, batched, served, reference")
+                    help="comma-separated parts to run: plain, batched, served, reference, dense")
+    ap.add_argument("--dense-steps", type=int, default=200)
     ap.add_argument("--reference-steps", type=int, default=100)
     ap.add_argument("--batched-no-compare", action="store_true",
                     help="batched leg without the one-domain-at-a-time comparison (for kernel traces)")
@@ -409,6 +436,9 @@ def main():
     if "reference" in legs:
         print("bench.py: reference benchmark workload", file=sys.stderr, flush=True)
         out["reference_benchmark"] = bench_reference(args.reference_steps, cpu=cpu, budget_s=args.cpu_budget)
+    if "dense" in legs:
+        print("bench.py: dense-agent workload", file=sys.stderr, flush=True)
+        out["dense_agents"] = bench_dense(args.dense_steps, cpu=cpu, budget_s=args.cpu_budget)
     if cpu and "plain" in legs:
         print("bench.py: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
