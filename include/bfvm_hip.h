@@ -305,9 +305,10 @@ typedef enum bfvm_x_sweep
  * streams, the y lines of the planes the first half covers starting as soon
  * as it is done (the x sweep's last partial round of waves then overlaps y
  * work instead of running alone); 2 = one launch per axis; 0 = the default:
- * halves when the x sweep needs more than one and at most three rounds of
- * the GPU's register file and the y lines hold two waves per SIMD (buckets
- * up to 208, or two lanes per line).  Neither setting changes the results.
+ * halves when the x lanes (unless the x sweep is the scan kernel) and the y
+ * lanes each need more than one round of the GPU's register file and the y
+ * lines hold two waves per SIMD (buckets up to 208, or two lanes per line).
+ * Neither setting changes the results.
  * bfvm_time_stages always times one launch per axis. */
 typedef struct bfvm_tuning
 {

@@ -37,7 +37,7 @@ def run(cmd, cwd=None):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
-HIP_SOURCES = ([f"bfvm_twist_{a}{h}.hip" for a in "yz" for h in "abcd"] +
+HIP_SOURCES = ([f"bfvm_twist_{a}{h}.hip" for a in "yz" for h in "abcde"] +
                [f"bfvm_sweeps_xt{s}{h}.hip" for s in (2, 4, 8) for h in "ab"] +
                [f"bfvm_sweeps_xg{s}{h}.hip" for s in range(1, 9) for h in "ab"] +
                [f"bfvm_sweeps_xgt{s}.hip" for s in range(1, 9)] +

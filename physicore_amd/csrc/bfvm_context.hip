@@ -705,6 +705,17 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepRows;
 				c->nmax[axis] = bucket;
 			}
+			// y lines of 209..256 voxels in 3-D domains (y not the last sweep,
+			// so no face or bulk stage): two lanes per line, two waves per SIMD
+			// instead of one (ms/step, one lane / two: 224^3 x 4 0.239 / 0.221,
+			// 240^3 x 4 0.265 / 0.251, 250^3 x 4 0.292 / 0.285, 216^3 x 3
+			// 0.181 / 0.172; results differ by rounding only, 4e-7 after 10
+			// steps)
+			else if (axis == 1 && c->dims == 3 && c->fp64 && twist_short_bucket(n))
+			{
+				c->variant[axis] = kSweepTwist;
+				c->nmax[axis] = twist_short_bucket(n);
+			}
 			else if (bucket)
 			{
 				c->variant[axis] = kSweepReg;
@@ -749,20 +760,25 @@ int build_tiles(bfvm_ctx* c)
 	}
 	// x in two halves (3-D domains; x tiles of rows or of lines, y tiles of
 	// columns): the last round of x waves then overlaps the y tiles of the
-	// first half's planes instead of running alone.  By default only where
-	// that was measured to pay (200^3 x 4 steps 3% faster, 400x400x100 x 2 7%):
-	// the x lanes fill more than one and at most three rounds of the register
-	// file at two waves per SIMD, and the y lines run two waves per SIMD too.
-	// Below one round the halves only add launches (a 176^3 x 4 step is 10%
-	// slower split); y lines of 209..256 voxels (one wave per SIMD) crowd out
-	// the second half (200x245x200 x 4: 6% slower)
+	// first half's planes instead of running alone.  By default where that
+	// was measured to pay: the x lanes (one or two per line; the scan kernel
+	// excepted, which loops over its rows) and the y lanes each fill more
+	// than one round of the register file at two waves per SIMD, and the y
+	// lines run two waves per SIMD (ms/step, one launch / halves: 200^3 x 4
+	// 0.159 / 0.151, 400x400x100 x 2 0.158 / 0.144, 300^3 x 1 (scan) 0.174 /
+	// 0.161, 256^3 x 8 0.674 / 0.662, 250^3 x 4 0.286 / 0.285).  Below one
+	// round the halves only add launches (176^3 x 4 0.090 / 0.095,
+	// 400x300x100 x 2 0.115 / 0.119, 300x120x120 x 1 0.053 / 0.059); y lines
+	// in one-lane buckets of 224..256 (one wave per SIMD) crowd out the second
+	// half (FP64 takes two lanes per line there)
 	c->split_x = c->split_y = 0;
 	const int vx = c->variant[0], vy = c->variant[1];
 	const int64_t lanes = c->n[1] * c->n[2] * c->S * (vx == kSweepXGT ? 2 : 1);
+	const int64_t ylanes = c->n[0] * c->n[2] * c->S * (vy == kSweepTwist ? 2 : 1);
 	const int64_t round = (int64_t)c->num_cu * 4 * 2 * 64;
 	bool want = c->tuning.split_xy == 1;
 	if (c->tuning.split_xy == 0)
-		want = lanes > round && lanes <= 3 * round && vx != kSweepScan &&
+		want = (vx == kSweepScan || lanes > round) && ylanes > round &&
 			   (vy == kSweepTwist || (vy == kSweepReg && c->nmax[1] <= 208));
 	if (want && c->dims == 3 && host_tiles[0].size() >= 2 &&
 		(vx == kSweepXG || vx == kSweepXGT || vx == kSweepXT || vx == kSweepReg || vx == kSweepScan) &&

@@ -60,6 +60,8 @@ kernel_fn pick_twist_z_a(int nmax, bool faces);
 kernel_fn pick_twist_z_b(int nmax, bool faces);
 kernel_fn pick_twist_z_c(int nmax, bool faces);
 kernel_fn pick_twist_z_d(int nmax, bool faces);
+kernel_fn pick_twist_y_e(int nmax, bool faces);
+kernel_fn pick_twist_z_e(int nmax, bool faces);
 kernel_fn pick_xscan(int S, int vpl, bool fp64);
 pipe_fn pick_pipe_x_s1(int nmax, bool faces);
 pipe_fn pick_pipe_x_s2(int nmax, bool faces);
@@ -204,10 +206,13 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, int fin, int S)
 	{
 		if (axis == 0 || !fp64)
 			return nullptr;
-		// instantiation groups of four buckets: 136..160, 168..192, 200..224, 232..256
-		const int g = nmax <= 160 ? 0 : nmax <= 192 ? 1 : nmax <= 224 ? 2 : 3;
-		static kernel_fn (*const y[4])(int, bool) = { pick_twist_y_a, pick_twist_y_b, pick_twist_y_c, pick_twist_y_d };
-		static kernel_fn (*const z[4])(int, bool) = { pick_twist_z_a, pick_twist_z_b, pick_twist_z_c, pick_twist_z_d };
+		// instantiation groups of buckets: 136..160, 168..192, 200..224, 232..256
+		// and, for lines of 209..256 voxels, 112..128
+		const int g = nmax <= 128 ? 4 : nmax <= 160 ? 0 : nmax <= 192 ? 1 : nmax <= 224 ? 2 : 3;
+		static kernel_fn (*const y[5])(int, bool) = { pick_twist_y_a, pick_twist_y_b, pick_twist_y_c, pick_twist_y_d,
+													   pick_twist_y_e };
+		static kernel_fn (*const z[5])(int, bool) = { pick_twist_z_a, pick_twist_z_b, pick_twist_z_c, pick_twist_z_d,
+													   pick_twist_z_e };
 		return axis == 1 ? y[g](nmax, faces) : z[g](nmax, faces);
 	}
 	// FP64 instantiations split at 128 lines (two translation units per axis)
@@ -231,6 +236,13 @@ int reg_bucket(int n)
 int twist_bucket(int n)
 {
 	if (n <= kRegMaxN || n > 2 * kRegMaxN)
+		return 0;
+	return (((n + 1) / 2 + 7) / 8) * 8;
+}
+
+int twist_short_bucket(int n)
+{
+	if (n <= kTwistShortMin || n > kRegMaxN)
 		return 0;
 	return (((n + 1) / 2 + 7) / 8) * 8;
 }
@@ -359,7 +371,7 @@ hipError_t init_sweep_kernels()
 				return e;
 		}
 	for (int axis = 1; axis < 3; axis++)
-		for (int nmax = twist_bucket(kRegMaxN + 1); nmax <= kRegMaxN; nmax += 8)
+		for (int nmax = twist_short_bucket(kTwistShortMin + 1); nmax <= kRegMaxN; nmax += 8)
 			for (int faces = 0; faces < 2; faces++)
 			{
 				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, faces ? kFinFaces : kFinNone, 1);

@@ -1,0 +1,25 @@
+// Instantiations of the twisted two-lane y-sweep, lines of 209..256 voxels
+// (half-line buckets of 8).
+#include "bfvm_twist_kernels.h"
+
+namespace bfvm {
+namespace sweep {
+
+kernel_fn pick_twist_y_e(int nmax, bool faces)
+{
+#define BFVM_BUCKET(N) \
+	case N:            \
+		return faces ? sweep_twist_kernel<1, N, true> : sweep_twist_kernel<1, N, false>;
+	switch (nmax)
+	{
+		BFVM_BUCKET(112)
+		BFVM_BUCKET(120)
+		BFVM_BUCKET(128)
+	default:
+		return nullptr;
+	}
+#undef BFVM_BUCKET
+}
+
+} // namespace sweep
+} // namespace bfvm

@@ -59,6 +59,11 @@ SHAPES = [
     # halves' bucket edges, the longest line
     (3, (5, 257, 3), 2), (3, (3, 401, 4), 3), (3, (4, 3, 512), 1), (3, (7, 5, 385), 2),
     (2, (9, 333, 1), 2),
+    # y lines of 209..256 in 3-D domains (two lanes per line, half-line
+    # buckets 112..128), and the same length as the last (2-D) axis, which
+    # keeps one lane per line
+    (3, (5, 209, 3), 2), (3, (4, 256, 3), 3), (3, (6, 230, 2), 1), (3, (3, 241, 4), 4),
+    (2, (5, 233, 1), 2),
 ]
 
 
@@ -235,11 +240,13 @@ def test_split_xy_same_results(grid, S, x_sweep):
     assert np.array_equal(out[0], out[1])
 
 
-# the default: halves when the x lanes fill more than one and at most three
-# rounds of the register file and the y lines run two waves per SIMD
-DEFAULT_SPLIT = [((200, 200, 200), 4, True), ((176, 176, 176), 4, False), ((200, 245, 200), 4, False),
-                 ((400, 400, 100), 2, True), ((250, 250, 250), 4, False), ((160, 160, 160), 8, True),
-                 ((128, 128, 128), 8, False)]
+# the default: halves when the x lanes (the scan kernel excepted) and the y
+# lanes each fill more than one round of the register file and the y lines run
+# two waves per SIMD
+DEFAULT_SPLIT = [((200, 200, 200), 4, True), ((176, 176, 176), 4, False), ((200, 245, 200), 4, True),
+                 ((400, 400, 100), 2, True), ((250, 250, 250), 4, True), ((160, 160, 160), 8, True),
+                 ((128, 128, 128), 8, False), ((300, 300, 300), 1, True), ((400, 300, 100), 2, False),
+                 ((300, 120, 120), 1, False)]
 
 
 @pytest.mark.parametrize("grid,S,split", DEFAULT_SPLIT)
