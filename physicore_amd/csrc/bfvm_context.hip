@@ -705,13 +705,14 @@ int build_tiles(bfvm_ctx* c)
 				c->variant[axis] = kSweepRows;
 				c->nmax[axis] = bucket;
 			}
-			// y lines of 209..256 voxels in 3-D domains (y not the last sweep,
-			// so no face or bulk stage): two lanes per line, two waves per SIMD
-			// instead of one (ms/step, one lane / two: 224^3 x 4 0.239 / 0.221,
-			// 240^3 x 4 0.265 / 0.251, 250^3 x 4 0.292 / 0.285, 216^3 x 3
-			// 0.181 / 0.172; results differ by rounding only, 4e-7 after 10
-			// steps)
-			else if (axis == 1 && c->dims == 3 && c->fp64 && twist_short_bucket(n))
+			// y/z lines of 209..256 voxels: two lanes per line, two waves per
+			// SIMD instead of one (ms/step, one lane / two, y lines: 224^3 x 4
+			// 0.239 / 0.221, 240^3 x 4 0.265 / 0.251, 250^3 x 4 0.292 / 0.285,
+			// 216^3 x 3 0.181 / 0.172; results differ by rounding only, 4e-7
+			// after 10 steps); the last sweep only without a bulk list (a
+			// uniform bulk rides on the one-lane kernel's stores, and a domain
+			// keeps its kernels whatever rates its bulk has)
+			else if (axis > 0 && c->fp64 && twist_short_bucket(n) && (axis < c->dims - 1 || c->bulk_count == 0))
 			{
 				c->variant[axis] = kSweepTwist;
 				c->nmax[axis] = twist_short_bucket(n);
@@ -2003,6 +2004,7 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 			}
 	}
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
+	const bool had_bulk = c->bulk_count > 0;
 	c->bulk_count = count;
 	c->bulk_runs = runs;
 	if (runs)
@@ -2028,7 +2030,10 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 		TRY_HIP(c, c->bulk_cell.alloc((size_t)count));
 		TRY_HIP(c, launch_bulk_map(c->bulk_vox.p, count, c->bulk_map.p, c->V, c->stream));
 	}
-	int rc = sync_domain(c);
+	// the last sweep's kernel depends on whether a bulk list exists
+	int rc = had_bulk != (count > 0) ? build_tiles(c) : BFVM_OK;
+	if (!rc)
+		rc = sync_domain(c);
 	return rc ? rc : rebuild_overlap(c);
 }
 

@@ -5,6 +5,8 @@ relative L-infinity error per substrate of 2e-6 (a few FP32 roundings); multi-st
 runs to the 1e-5 bar of the benchmark.  Cell sums are FP64 on both sides: the
 sorted reduction must match the oracle bit for bit, the atomic one to 1e-13.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -63,7 +65,7 @@ SHAPES = [
     # buckets 112..128), and the same length as the last (2-D) axis, which
     # keeps one lane per line
     (3, (5, 209, 3), 2), (3, (4, 256, 3), 3), (3, (6, 230, 2), 1), (3, (3, 241, 4), 4),
-    (2, (5, 233, 1), 2),
+    (2, (5, 233, 1), 2), (3, (4, 3, 230), 2), (3, (5, 4, 209), 3),
 ]
 
 
@@ -526,9 +528,10 @@ def test_dirichlet_precedence_interior_over_faces_random():
     assert rel_linf(d.field(), s.field).max() < 3e-6
 
 
-@pytest.mark.parametrize("dims,grid", [(3, (6, 5, 301)), (2, (7, 333, 1)), (3, (4, 290, 3))])
+@pytest.mark.parametrize("dims,grid", [(3, (6, 5, 301)), (2, (7, 333, 1)), (3, (4, 290, 3)), (3, (6, 5, 240)),
+                                       (2, (7, 220, 1)), (3, (5, 231, 4))])
 def test_long_lines_with_faces(dims, grid):
-    """Lines of 257..512 voxels (two lanes per line) in the sweep that applies
+    """Lines of 209..512 voxels (two lanes per line) in the sweep that applies
     the Dirichlet faces (z in 3-D, y in 2-D), every face set for some
     substrate, and lines lying in faces of the other axes."""
     rng = np.random.default_rng(sum(grid))
@@ -1050,6 +1053,36 @@ def test_uniform_bulk_applied_by_last_sweep(dims, grid, S, shape, nearest):
     keep[j] = False
     assert np.array_equal(fa[keep], fb[keep])
     assert not np.array_equal(fa[j], fb[j])
+
+
+def test_last_sweep_kernel_follows_the_bulk():
+    """z lines of 209..256 voxels take two lanes per line while the domain has
+    no bulk list, one lane per line (which applies a uniform bulk as it
+    stores) with one: the kernel changes with the list, and both steps match
+    the oracle"""
+    from physicore_amd import workloads as wl
+
+    w, _, _ = _uniform_bulk_workloads(3, (6, 5, 240), 2, seed=5, shape="dense")
+    wn = dataclasses.replace(w, bulk=None)
+    d = wl.make_domain(wn)
+    assert d.sweep_plan()["kernels"][2] == 5
+    d.step(2)
+    p, field, dirs, bulk, ag = sc.oracle_inputs(wn)
+    s = orc.Solver(p, field.astype(np.float32).astype(np.float64), dirichlet=dirs, bulk=bulk, agents=ag)
+    s.solve(2)
+    assert rel_linf(d.field(), s.field).max() < 1e-5
+    d.set_bulk(*w.bulk)
+    assert d.sweep_plan()["kernels"][2] == 1
+    d2 = wl.make_domain(w)
+    assert d2.sweep_plan()["kernels"][2] == 1
+    d2.step(3)
+    p, field, dirs, bulk, ag = sc.oracle_inputs(w)
+    s = orc.Solver(p, field.astype(np.float32).astype(np.float64), dirichlet=dirs, bulk=bulk, agents=ag)
+    s.solve(3)
+    assert rel_linf(d2.field(), s.field).max() < 1e-5
+    # an empty list: two lanes again
+    d.set_bulk(np.zeros(0, dtype=np.int64), np.zeros((0, 2)), np.zeros((0, 2)), np.zeros((0, 2)))
+    assert d.sweep_plan()["kernels"][2] == 5
 
 
 @pytest.mark.parametrize("shape", ["dense", "ball"])
