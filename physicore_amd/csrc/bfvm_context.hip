@@ -365,6 +365,7 @@ struct bfvm_ctx
 	dvec<int32_t> cell_voxel, owner_of, ballot, owners, flags;
 	dvec<double> num, den, fac, rnum, rden, rfac, dnew, pnum, pden, pfac;
 	dvec<int32_t> pvox, slot_of;
+	dvec<uint8_t> slot_alone, joined;
 	dvec<TileRef> cell_tiles, int_tiles; // sparse tiles of the owner list / of all agents
 	int n_cell_tiles = 0, n_int_tiles = 0;
 	dvec<uint32_t> keys, vals, tkeys, tvals;
@@ -548,6 +549,8 @@ int sync_domain(bfvm_ctx* c)
 	d.rden = c->rden.p;
 	d.rfac = c->rfac.p;
 	d.dnew = c->dnew.p;
+	d.slot_alone = c->slot_alone.p;
+	d.joined = c->joined.p;
 	d.owners = c->owners.p;
 	d.pvox = c->pvox.p;
 	d.slot_of = c->slot_of.p;
@@ -2087,6 +2090,8 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->pvox.alloc(n));
 	TRY_HIP(c, c->cell_in_bulk.alloc(n));
 	TRY_HIP(c, c->slot_of.alloc(n));
+	TRY_HIP(c, c->slot_alone.alloc(n));
+	TRY_HIP(c, c->joined.alloc(n));
 	// the recompute's sort buffers (the owner list has a slot per agent)
 	TRY_HIP(c, c->keys.alloc(n));
 	TRY_HIP(c, c->vals.alloc(n));

@@ -87,6 +87,11 @@ struct DomainDev
 	double* pden;
 	double* pfac;
 	double* dnew;
+	// with a shared voxel somewhere and tracking on: per slot, whether the
+	// owner's voxel holds the owner only (its agent update then runs in the
+	// apply); per agent (atomic mode), whether another agent joined its voxel
+	uint8_t* slot_alone; // [n]
+	uint8_t* joined;	 // [n]
 	int32_t* flags;	 // [0] conflict, [1] agent outside grid, [2] owner count
 	int32_t compute_internalized;
 	int32_t pad1;

@@ -92,6 +92,7 @@ __global__ void cells_bin_kernel(const DomainDev* __restrict__ dp, int64_t n, in
 	d.rfac[k] = 0;
 	if (k != i * S)
 		return;
+	d.joined[i] = 0;
 	if (!d.is_active[i])
 	{
 		d.cell_voxel[i] = -1;
@@ -161,6 +162,8 @@ __global__ __launch_bounds__(256) void cells_ballot_kernel(const DomainDev* __re
 		d.owner_of[i] = owner;
 		if (!head)
 			d.flags[0] = 1;
+		if (owner != (int)i)
+			d.joined[owner] = 1; // every writer stores 1
 	}
 	const int S = d.S;
 	const bool self = key >= 0 && head && won; // this lane is the owner
@@ -238,6 +241,7 @@ __global__ void cells_segment_reduce_kernel(const DomainDev* __restrict__ dp, co
 		q1++;
 	if (q1 - p > 1)
 		d.flags[0] = 1;
+	d.slot_alone[head_scan[p]] = q1 - p == 1;
 	for (int64_t q = p; q < q1; q++)
 		d.owner_of[vals[q]] = owner;
 	for (int s = 0; s < S; s++)
@@ -306,6 +310,8 @@ __global__ void cells_pack_kernel(const DomainDev* __restrict__ dp, int64_t n, i
 	{
 		d.pvox[k] = d.cell_voxel[o];
 		d.slot_of[o] = (int32_t)k;
+		if (own_terms)
+			d.slot_alone[k] = !d.joined[o];
 	}
 	if (own_terms)
 	{
@@ -344,6 +350,16 @@ __global__ void cells_agent_slots_kernel(const DomainDev* __restrict__ dp, int64
 // so every listed owner is active.
 // The cell update of apply entry k, substrate s, from the density d0 at its
 // voxel; returns the FP32 value to store there.
+// The per-agent internalized update with a shared voxel somewhere
+// (cell_solver.cpp:229-243): I - V*(num - D_new*den + fac), the agent's own
+// terms.  One function for both callers, so both compile to the same
+// operations.
+__device__ __forceinline__ double agent_internalized(double I, double V, double num, double dnew, double den,
+													 double fac)
+{
+	return I - V * (num - dnew * den + fac);
+}
+
 __device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, double d0, uint32_t seed, bool fused,
 											 bool nearest)
 {
@@ -354,7 +370,18 @@ __device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, d
 	if (fused)
 		d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
 	else if (d.compute_internalized)
-		d.dnew[q] = d1; // read by the per-agent update (tracking on, shared voxels)
+	{
+		if (d.slot_alone[k])
+		{
+			// the owner alone in its voxel: its own num and fac are the
+			// slot's sums (num + 0 or num itself), its den is read back
+			const int64_t qa = (int64_t)d.owners[k] * S + s;
+			d.internalized[qa] = agent_internalized(d.internalized[qa], d.voxel_volume, d.pnum[q], d1, d.den[qa],
+													d.pfac[q]);
+		}
+		else
+			d.dnew[q] = d1; // read by the per-agent update
+	}
 	return sr_round(d1, mix32((uint32_t)e ^ seed), nearest);
 }
 
@@ -466,10 +493,11 @@ __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, co
 	{
 		const int i = tr.a + t / S, s = t % S;
 		const int64_t slot = d.owner_of[i]; // the owner's slot (-1: inactive agent)
-		if (slot < 0)
+		if (slot < 0 || d.slot_alone[slot]) // an owner alone was updated by the apply
 			continue;
 		const int64_t q = (int64_t)i * S + s;
-		d.internalized[q] -= d.voxel_volume * (d.num[q] - d.dnew[slot * S + s] * d.den[q] + d.fac[q]);
+		d.internalized[q] = agent_internalized(d.internalized[q], d.voxel_volume, d.num[q], d.dnew[slot * S + s],
+											   d.den[q], d.fac[q]);
 	}
 }
 
