@@ -366,6 +366,7 @@ struct bfvm_ctx
 	dvec<double> num, den, fac, rnum, rden, rfac, dnew, pnum, pden, pfac;
 	dvec<int32_t> pvox, slot_of;
 	dvec<uint8_t> slot_alone, joined;
+	dvec<int32_t> shared_list;
 	dvec<TileRef> cell_tiles, int_tiles; // sparse tiles of the owner list / of all agents
 	int n_cell_tiles = 0, n_int_tiles = 0;
 	dvec<uint32_t> keys, vals, tkeys, tvals;
@@ -551,6 +552,7 @@ int sync_domain(bfvm_ctx* c)
 	d.dnew = c->dnew.p;
 	d.slot_alone = c->slot_alone.p;
 	d.joined = c->joined.p;
+	d.shared_list = c->shared_list.p;
 	d.owners = c->owners.p;
 	d.pvox = c->pvox.p;
 	d.slot_of = c->slot_of.p;
@@ -994,6 +996,8 @@ int recompute(bfvm_ctx* c)
 	}
 	TRY_HIP(c, launch_cells_pack(c->ddom.p, n, c->S, c->reduce_mode == BFVM_REDUCE_ATOMIC, s));
 	TRY_HIP(c, launch_cells_agent_slots(c->ddom.p, n, s));
+	if (c->internalized)
+		TRY_HIP(c, launch_cells_shared_list(c->ddom.p, n, s));
 	c->grid_check_pending = true;
 	return rebuild_overlap(c);
 }
@@ -2092,6 +2096,7 @@ int bfvm_set_agents(bfvm_ctx* c, int64_t n, const double* positions, const doubl
 	TRY_HIP(c, c->slot_of.alloc(n));
 	TRY_HIP(c, c->slot_alone.alloc(n));
 	TRY_HIP(c, c->joined.alloc(n));
+	TRY_HIP(c, c->shared_list.alloc(n));
 	// the recompute's sort buffers (the owner list has a slot per agent)
 	TRY_HIP(c, c->keys.alloc(n));
 	TRY_HIP(c, c->vals.alloc(n));

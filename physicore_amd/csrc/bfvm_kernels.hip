@@ -480,8 +480,26 @@ __global__ void bulk_overlap_kernel(const DomainDev* __restrict__ dom, const int
 		d.bulk_cell[b] = (int32_t)k;
 }
 
+// The per-agent update's work list (after the owner slots are known): every
+// agent whose voxel holds more than one agent; the owners alone in theirs
+// take the update in the apply.  The list order does not matter (each entry
+// updates its own amounts).
+__global__ void cells_shared_list_kernel(const DomainDev* __restrict__ dp, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const DomainDev& d = *dp;
+	const int slot = d.owner_of[i];
+	if (slot >= 0 && !d.slot_alone[slot])
+		d.shared_list[atomicAdd(&d.flags[3], 1)] = (int32_t)i;
+}
+
 // compute_internalized (cell_solver.cpp:131-144, :229-243):
-// I -= V*(num - D_new*den + fac) for every active agent, when voxels are shared.
+// I -= V*(num - D_new*den + fac) for every active agent, when voxels are
+// shared: the agents of the shared voxels from the work list (tiles over list
+// positions, stopping at its length on the device); the owners alone in their
+// voxels were updated by the apply.
 __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
 {
 	const TileRef tr = tiles[blockIdx.x];
@@ -489,12 +507,11 @@ __global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, co
 	if (!d.flags[0]) // no shared voxel: the apply updated the owners' amounts
 		return;
 	const int S = d.S;
-	for (int t = threadIdx.x; t < tr.b * S; t += blockDim.x)
+	const int cnt = max(0, min(tr.b, d.flags[3] - tr.a));
+	for (int t = threadIdx.x; t < cnt * S; t += blockDim.x)
 	{
-		const int i = tr.a + t / S, s = t % S;
-		const int64_t slot = d.owner_of[i]; // the owner's slot (-1: inactive agent)
-		if (slot < 0 || d.slot_alone[slot]) // an owner alone was updated by the apply
-			continue;
+		const int i = d.shared_list[tr.a + t / S], s = t % S;
+		const int64_t slot = d.owner_of[i];
 		const int64_t q = (int64_t)i * S + s;
 		d.internalized[q] = agent_internalized(d.internalized[q], d.voxel_volume, d.num[q], d.dnew[slot * S + s],
 											   d.den[q], d.fac[q]);
@@ -853,6 +870,11 @@ hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, int S, bool own_te
 hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream)
 {
 	return launch1d(cells_agent_slots_kernel, n, 256, stream, dom, n);
+}
+
+hipError_t launch_cells_shared_list(const DomainDev* dom, int64_t n, hipStream_t stream)
+{
+	return launch1d(cells_shared_list_kernel, n, 256, stream, dom, n);
 }
 
 hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream)
