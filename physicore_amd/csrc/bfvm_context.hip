@@ -969,7 +969,7 @@ int recompute(bfvm_ctx* c)
 	// everything below is enqueued on the stream: the owner count and the
 	// flags stay on the device (the sort buffers were sized by bfvm_set_agents)
 	hipStream_t s = c->stream;
-	TRY_HIP(c, hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int32_t), s));
+	TRY_HIP(c, hipMemsetAsync(c->flags.p, 0, kCellFlags * sizeof(int32_t), s));
 	if (c->reduce_mode == BFVM_REDUCE_ATOMIC)
 	{
 		TRY_HIP(c, launch_cells_bin(c->ddom.p, n, c->S, true, s));
@@ -996,8 +996,12 @@ int recompute(bfvm_ctx* c)
 	}
 	TRY_HIP(c, launch_cells_pack(c->ddom.p, n, c->S, c->reduce_mode == BFVM_REDUCE_ATOMIC, s));
 	TRY_HIP(c, launch_cells_agent_slots(c->ddom.p, n, s));
+	// (the scan reuses the sorted path's head scan buffer, free by now)
 	if (c->internalized)
-		TRY_HIP(c, launch_cells_shared_list(c->ddom.p, n, s));
+	{
+		TRY_HIP(c, launch_cells_owner_order(c->ddom.p, n, s));
+		TRY_HIP(c, launch_cells_shared_list(c->ddom.p, n, c->head_scan.p, c->scan_tmp.p, s));
+	}
 	c->grid_check_pending = true;
 	return rebuild_overlap(c);
 }
@@ -1835,8 +1839,8 @@ int bfvm_create(const bfvm_domain_desc* desc, bfvm_ctx** out)
 		if ((e = c->coef_map.upload(m.data(), m.size(), c->stream)) != hipSuccess)
 			return bail(c->hip_fail(e, "coefficient map"));
 	}
-	if ((e = c->flags.alloc(4)) != hipSuccess ||
-		(e = hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int32_t), c->stream)) != hipSuccess ||
+	if ((e = c->flags.alloc(kCellFlags)) != hipSuccess ||
+		(e = hipMemsetAsync(c->flags.p, 0, kCellFlags * sizeof(int32_t), c->stream)) != hipSuccess ||
 		(e = hipStreamSynchronize(c->stream)) != hipSuccess)
 		return bail(c->hip_fail(e, "flags"));
 	int rc = build_tiles(c);

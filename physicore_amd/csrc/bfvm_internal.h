@@ -92,10 +92,11 @@ struct DomainDev
 	// apply); per agent (atomic mode), whether another agent joined its voxel
 	uint8_t* slot_alone; // [n]
 	uint8_t* joined;	 // [n]
-	// (tracking on) the agents whose voxel is shared, in no particular order:
-	// the per-agent update's work list, flags[3] entries
+	// (tracking on) the agents whose voxel is shared, in agent order: the
+	// per-agent update's work list, flags[3] entries
 	int32_t* shared_list; // [n]
-	int32_t* flags;	 // [0] conflict, [1] agent outside grid, [2] owner count, [3] shared_list entries
+	int32_t* flags; // kCellFlags: [0] conflict, [1] agent outside grid, [2] owner count,
+					// [3] shared_list entries, [4] owner list entries below their predecessor's agent
 	int32_t compute_internalized;
 	int32_t pad1;
 };
@@ -410,7 +411,10 @@ hipError_t launch_cells_ballot(const DomainDev* dom, int64_t n, hipStream_t stre
 hipError_t launch_cells_pack(const DomainDev* dom, int64_t n, int S, bool own_terms, hipStream_t stream);
 // After the pack: owner_of[i] becomes the slot of agent i's owner (-1 stays).
 hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t stream);
-hipError_t launch_cells_shared_list(const DomainDev* dom, int64_t n, hipStream_t stream);
+constexpr int kCellFlags = 8; // entries of DomainDev::flags
+hipError_t launch_cells_owner_order(const DomainDev* dom, int64_t n, hipStream_t stream);
+// pos: n entries of scratch (the scan), tmp: scan_tmp_entries(n)
+hipError_t launch_cells_shared_list(const DomainDev* dom, int64_t n, int32_t* pos, int32_t* tmp, hipStream_t stream);
 // atomic mode: keys/vals = (voxel, agent) of each owner, key V for the other
 // agents, to sort the owners into voxel order (radix_sort_pairs); then the
 // owner count (flags[2]) from the sorted keys

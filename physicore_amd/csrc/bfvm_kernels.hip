@@ -360,6 +360,16 @@ __device__ __forceinline__ double agent_internalized(double I, double V, double 
 	return I - V * (num - dnew * den + fac);
 }
 
+// Whether the owner list (in voxel order) is also nearly in agent order, at
+// most one entry in 16 below its predecessor's agent: then the owners' agent
+// arrays read in list order are contiguous, and the owners alone in their
+// voxel take their per-agent update in the apply; otherwise those reads would
+// scatter, and the per-agent pass (in agent order) takes them.
+__device__ __forceinline__ bool owners_in_agent_order(const DomainDev& d)
+{
+	return d.flags[4] * 16 <= d.flags[2];
+}
+
 __device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, double d0, uint32_t seed, bool fused,
 											 bool nearest)
 {
@@ -371,7 +381,7 @@ __device__ __forceinline__ float cell_update(const DomainDev& d, int k, int s, d
 		d.internalized[(int64_t)d.owners[k] * S + s] += d.voxel_volume * (d0 - d1);
 	else if (d.compute_internalized)
 	{
-		if (d.slot_alone[k])
+		if (d.slot_alone[k] && owners_in_agent_order(d))
 		{
 			// the owner alone in its voxel: its own num and fac are the
 			// slot's sums (num + 0 or num itself), its den is read back
@@ -481,18 +491,45 @@ __global__ void bulk_overlap_kernel(const DomainDev* __restrict__ dom, const int
 }
 
 // The per-agent update's work list (after the owner slots are known): every
-// agent whose voxel holds more than one agent; the owners alone in theirs
-// take the update in the apply.  The list order does not matter (each entry
-// updates its own amounts).
-__global__ void cells_shared_list_kernel(const DomainDev* __restrict__ dp, int64_t n)
+// agent whose voxel holds more than one agent, in agent order (the update
+// then reads the agent arrays in order); the owners alone in theirs take the
+// update in the apply.  Built as a compaction: the membership of each agent,
+// an exclusive scan of it, then the scatter, which also stores the length.
+__device__ __forceinline__ int in_shared_voxel(const DomainDev& d, int64_t i)
+{
+	const int slot = d.owner_of[i];
+	return slot >= 0 && !(d.slot_alone[slot] && owners_in_agent_order(d));
+}
+
+// Counts the owner list entries whose agent precedes the previous entry's.
+__global__ void cells_owner_order_kernel(const DomainDev* __restrict__ dp, int64_t n)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const DomainDev& d = *dp;
+	const bool down = k >= 1 && k < n && k < d.flags[2] && d.owners[k] < d.owners[k - 1];
+	const unsigned long long m = __ballot(down);
+	if ((threadIdx.x & (kWave - 1)) == 0 && m)
+		atomicAdd(&d.flags[4], __popcll(m));
+}
+
+__global__ void cells_shared_mark_kernel(const DomainDev* __restrict__ dp, int64_t n, int32_t* mark)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		mark[i] = in_shared_voxel(*dp, i);
+}
+
+__global__ void cells_shared_scatter_kernel(const DomainDev* __restrict__ dp, int64_t n, const int32_t* __restrict__ pos)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	const DomainDev& d = *dp;
-	const int slot = d.owner_of[i];
-	if (slot >= 0 && !d.slot_alone[slot])
-		d.shared_list[atomicAdd(&d.flags[3], 1)] = (int32_t)i;
+	const int m = in_shared_voxel(d, i);
+	if (m)
+		d.shared_list[pos[i]] = (int32_t)i;
+	if (i == n - 1)
+		d.flags[3] = pos[i] + m;
 }
 
 // compute_internalized (cell_solver.cpp:131-144, :229-243):
@@ -500,21 +537,27 @@ __global__ void cells_shared_list_kernel(const DomainDev* __restrict__ dp, int64
 // shared: the agents of the shared voxels from the work list (tiles over list
 // positions, stopping at its length on the device); the owners alone in their
 // voxels were updated by the apply.
-__global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles)
+// The workgroups loop over the tiles (the list is usually far shorter than
+// the agent slots the tiles cover, so most tiles are empty).
+__global__ void cells_internalized_kernel(const DomainDev* __restrict__ doms, const TileRef* __restrict__ tiles,
+										  int ntiles)
 {
-	const TileRef tr = tiles[blockIdx.x];
-	const DomainDev& d = doms[tr.dom];
-	if (!d.flags[0]) // no shared voxel: the apply updated the owners' amounts
-		return;
-	const int S = d.S;
-	const int cnt = max(0, min(tr.b, d.flags[3] - tr.a));
-	for (int t = threadIdx.x; t < cnt * S; t += blockDim.x)
+	for (int b = blockIdx.x; b < ntiles; b += gridDim.x)
 	{
-		const int i = d.shared_list[tr.a + t / S], s = t % S;
-		const int64_t slot = d.owner_of[i];
-		const int64_t q = (int64_t)i * S + s;
-		d.internalized[q] = agent_internalized(d.internalized[q], d.voxel_volume, d.num[q], d.dnew[slot * S + s],
-											   d.den[q], d.fac[q]);
+		const TileRef tr = tiles[b];
+		const DomainDev& d = doms[tr.dom];
+		if (!d.flags[0]) // no shared voxel: the apply updated the owners' amounts
+			continue;
+		const int S = d.S;
+		const int cnt = max(0, min(tr.b, d.flags[3] - tr.a));
+		for (int t = threadIdx.x; t < cnt * S; t += blockDim.x)
+		{
+			const int i = d.shared_list[tr.a + t / S], s = t % S;
+			const int64_t slot = d.owner_of[i];
+			const int64_t q = (int64_t)i * S + s;
+			d.internalized[q] = agent_internalized(d.internalized[q], d.voxel_volume, d.num[q],
+												   d.dnew[slot * S + s], d.den[q], d.fac[q]);
+		}
 	}
 }
 
@@ -847,7 +890,8 @@ hipError_t launch_cells_internalized(const DomainDev* doms, const TileRef* tiles
 {
 	if (ntiles <= 0)
 		return hipSuccess;
-	hipLaunchKernelGGL(cells_internalized_kernel, dim3(ntiles), dim3(256), 0, stream, doms, tiles);
+	hipLaunchKernelGGL(cells_internalized_kernel, dim3(std::min(ntiles, 2048)), dim3(256), 0, stream, doms, tiles,
+					   ntiles);
 	return hipGetLastError();
 }
 
@@ -872,9 +916,21 @@ hipError_t launch_cells_agent_slots(const DomainDev* dom, int64_t n, hipStream_t
 	return launch1d(cells_agent_slots_kernel, n, 256, stream, dom, n);
 }
 
-hipError_t launch_cells_shared_list(const DomainDev* dom, int64_t n, hipStream_t stream)
+hipError_t launch_cells_owner_order(const DomainDev* dom, int64_t n, hipStream_t stream)
 {
-	return launch1d(cells_shared_list_kernel, n, 256, stream, dom, n);
+	return launch1d(cells_owner_order_kernel, n, 256, stream, dom, n);
+}
+
+hipError_t launch_cells_shared_list(const DomainDev* dom, int64_t n, int32_t* pos, int32_t* tmp, hipStream_t stream)
+{
+	if (n <= 0)
+		return hipSuccess;
+	hipError_t e = launch1d(cells_shared_mark_kernel, n, 256, stream, dom, n, pos);
+	if (e == hipSuccess)
+		e = exclusive_scan(pos, tmp, n, stream);
+	if (e == hipSuccess)
+		e = launch1d(cells_shared_scatter_kernel, n, 256, stream, dom, n, (const int32_t*)pos);
+	return e;
 }
 
 hipError_t launch_copy_field(const float* src, float* dst, int64_t n, int num_cu, hipStream_t stream)
