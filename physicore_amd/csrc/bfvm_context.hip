@@ -713,11 +713,18 @@ int build_tiles(bfvm_ctx* c)
 			// y/z lines of 209..256 voxels: two lanes per line, two waves per
 			// SIMD instead of one (ms/step, one lane / two, y lines: 224^3 x 4
 			// 0.239 / 0.221, 240^3 x 4 0.265 / 0.251, 250^3 x 4 0.292 / 0.285,
-			// 216^3 x 3 0.181 / 0.172; results differ by rounding only, 4e-7
-			// after 10 steps); the last sweep only without a bulk list (a
-			// uniform bulk rides on the one-lane kernel's stores, and a domain
-			// keeps its kernels whatever rates its bulk has)
-			else if (axis > 0 && c->fp64 && twist_short_bucket(n) && (axis < c->dims - 1 || c->bulk_count == 0))
+			// 216^3 x 3 0.181 / 0.172); lines of 137..208 when one lane per
+			// line needs more than one and at most two rounds of the register
+			// file at two waves per SIMD (three with half lines: 192^3 x 4
+			// 0.134 / 0.128, 144^3 x 8 0.115 / 0.110, 160^3 x 8 0.144 / 0.138;
+			// within one round the extra waves only cost, 176^3 x 4 0.089 /
+			// 0.096, and at 200^3 x 8, 2.4 rounds, 0.249 / 0.252).  Results
+			// differ by rounding only (4e-7 after 10 steps).  The last sweep
+			// only without a bulk list (a uniform bulk rides on the one-lane
+			// kernel's stores, and a domain keeps its kernels whatever rates
+			// its bulk has)
+			else if (axis > 0 && c->fp64 && twist_short_bucket(n) && (axis < c->dims - 1 || c->bulk_count == 0) &&
+					 (n > 208 || (lines > (int64_t)c->num_cu * 512 && lines <= (int64_t)c->num_cu * 1024)))
 			{
 				c->variant[axis] = kSweepTwist;
 				c->nmax[axis] = twist_short_bucket(n);

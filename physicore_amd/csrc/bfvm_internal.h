@@ -282,9 +282,10 @@ int reg_bucket(int n);
 // Half-line bucket of the twisted kernels (lines of kRegMaxN+1 .. 2*kRegMaxN):
 // ceil(n/2) rounded up to 8, or 0 outside that range.
 int twist_bucket(int n);
-// The same for y/z lines of kTwistShortMin+1 .. kRegMaxN, whose one-lane
-// buckets (224..256) need more than 256 registers (one wave per SIMD).
-constexpr int kTwistShortMin = 208;
+// The same for y/z lines of kTwistShortMin+1 .. kRegMaxN: one-lane buckets
+// of 224..256 need more than 256 registers (one wave per SIMD); from 144 the
+// half lines fit three waves per SIMD against two.
+constexpr int kTwistShortMin = 136;
 int twist_short_bucket(int n);
 constexpr int kTwistColsPerTile = kSweepThreads / 2; // two lanes per column
 // Row bucket of the chunked x kernel: n rounded up to 8, and up to 32 when

@@ -1,7 +1,8 @@
 // bfvm_twist_kernels.h -- y and z sweeps of lines of 257..512 voxels in
 // registers, two lanes per line (device template; instantiated by
-// bfvm_twist_{y,z}{a,b,c,d}.hip), and of lines of 209..256 voxels, whose
-// one-lane kernels need more than 256 registers (bfvm_twist_{y,z}e.hip).
+// bfvm_twist_{y,z}{a,b,c,d}.hip), and of lines of 137..256 voxels, whose
+// one-lane kernels need more than 256 registers (209..256) or hold fewer waves
+// per SIMD than half lines do (bfvm_twist_{y,z}e.hip).
 //
 // One lane cannot hold such a line (256 VGPRs), so each line is split at
 // m = ceil(n/2) and solved by a twisted factorization: the "top" lane

@@ -1,4 +1,4 @@
-// Instantiations of the twisted two-lane z-sweep, lines of 209..256 voxels
+// Instantiations of the twisted two-lane z-sweep, lines of 137..256 voxels
 // (half-line buckets of 8).
 #include "bfvm_twist_kernels.h"
 
@@ -12,6 +12,11 @@ kernel_fn pick_twist_z_e(int nmax, bool faces)
 		return faces ? sweep_twist_kernel<2, N, true> : sweep_twist_kernel<2, N, false>;
 	switch (nmax)
 	{
+		BFVM_BUCKET(72)
+		BFVM_BUCKET(80)
+		BFVM_BUCKET(88)
+		BFVM_BUCKET(96)
+		BFVM_BUCKET(104)
 		BFVM_BUCKET(112)
 		BFVM_BUCKET(120)
 		BFVM_BUCKET(128)

@@ -1094,6 +1094,22 @@ def test_uniform_bulk_applied_by_last_sweep(dims, grid, S, shape, nearest):
     assert not np.array_equal(fa[j], fb[j])
 
 
+@pytest.mark.parametrize("grid,S,kernels", [((100, 144, 170), 8, [6, 5, 1]), ((170, 100, 190), 8, [6, 1, 5])])
+def test_two_lane_mid_lines_past_one_round(grid, S, kernels):
+    """y or z lines of 137..208 voxels whose count needs more than one round
+    of the register file (256 CUs) at one lane per line take two lanes per
+    line; one step matches the oracle (the other axis, within one round,
+    keeps one lane per line)"""
+    rng = np.random.default_rng(sum(grid) + S)
+    p = orc.Problem(3, grid, (20, 20, 20), 0.01, 10 ** rng.uniform(0, 5, S), rng.uniform(0.01, 1, S))
+    f0 = rng.uniform(0, 1, p.shape)
+    d = gpu_domain(p, field=f0)
+    assert d.sweep_plan()["kernels"] == kernels
+    want = p.diffuse(f0.astype(np.float32).astype(np.float64))
+    d.step(1)
+    assert rel_linf(d.field(), want).max() < 2e-6
+
+
 def test_last_sweep_kernel_follows_the_bulk():
     """z lines of 209..256 voxels take two lanes per line while the domain has
     no bulk list, one lane per line (which applies a uniform bulk as it
