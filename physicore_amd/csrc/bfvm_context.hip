@@ -346,6 +346,10 @@ struct bfvm_ctx
 	// while storing when that sweep is a register y/z kernel and no interior
 	// Dirichlet stage sits between them
 	bool bulk_runs = false;
+	// the list's voxels alone (whatever the rates) have that shape: the last
+	// sweep then stays on one lane per line, so that a domain keeps its
+	// kernels when only the rates change
+	bool bulk_layout = false;
 	bool bulk_fused = false;
 	dvec<double> bulk_u;
 	dvec<int32_t> bulk_run;
@@ -720,11 +724,14 @@ int build_tiles(bfvm_ctx* c)
 			// within one round the extra waves only cost, 176^3 x 4 0.089 /
 			// 0.096, and at 200^3 x 8, 2.4 rounds, 0.249 / 0.252).  Results
 			// differ by rounding only (4e-7 after 10 steps).  The last sweep
-			// only without a bulk list (a uniform bulk rides on the one-lane
-			// kernel's stores, and a domain keeps its kernels whatever rates
-			// its bulk has)
-			else if (axis > 0 && c->fp64 && twist_short_bucket(n) && (axis < c->dims - 1 || c->bulk_count == 0) &&
-					 (n > 208 || (lines > (int64_t)c->num_cu * 512 && lines <= (int64_t)c->num_cu * 1024)))
+			// (with the faces) only for lines of 209..256 (200^3 x 4 with an
+			// x-min face: z 46.4 us back to back on one lane, 47.7 on two), and
+			// only when no bulk list has the shape a uniform bulk riding on
+			// the one-lane kernel's stores needs (whatever its rates, so that
+			// the kernels do not change with them)
+			else if (axis > 0 && c->fp64 && twist_short_bucket(n) &&
+					 (axis < c->dims - 1 ? n > 208 || (lines > (int64_t)c->num_cu * 512 && lines <= (int64_t)c->num_cu * 1024)
+										 : n > 208 && !c->bulk_layout))
 			{
 				c->variant[axis] = kSweepTwist;
 				c->nmax[axis] = twist_short_bucket(n);
@@ -1994,11 +2001,11 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 	// less than a sparse launch over a dense list (250^3 x 4, every voxel:
 	// +35 us against 480 us), more than one over a small ball (200^3 x 4,
 	// 0.8% of the voxels: 5.7 us)
-	bool runs = count > 0 && c->dims >= 2 && count * 8 >= c->V;
-	for (int64_t k = S; runs && k < count * S; k++)
-		runs = a[k] == a[k % S] && b[k] == b[k % S];
+	// (the layout alone, whatever the rates, also decides the last sweep's
+	// kernel: one lane per line where such a bulk could ride on it)
+	bool layout = count > 0 && c->dims >= 2 && count * 8 >= c->V;
 	std::vector<int32_t> run;
-	if (runs)
+	if (layout)
 	{
 		const int la = c->dims - 1;
 		const int64_t plane = c->n[0] * (la == 2 ? c->n[1] : 1); // voxels per position of the last axis
@@ -2013,18 +2020,22 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 			cnt[col]++;
 		}
 		run.assign(2 * cols, 0);
-		for (int64_t q = 0; runs && q < cols; q++)
+		for (int64_t q = 0; layout && q < cols; q++)
 			if (cnt[q])
 			{
-				runs = hi[q] - lo[q] + 1 == cnt[q];
+				layout = hi[q] - lo[q] + 1 == cnt[q];
 				run[2 * q] = lo[q];
 				run[2 * q + 1] = hi[q] + 1;
 			}
 	}
+	bool runs = layout;
+	for (int64_t k = S; runs && k < count * S; k++)
+		runs = a[k] == a[k % S] && b[k] == b[k % S];
 	TRY_HIP(c, hipStreamSynchronize(c->stream));
-	const bool had_bulk = c->bulk_count > 0;
+	const bool had_layout = c->bulk_layout;
 	c->bulk_count = count;
 	c->bulk_runs = runs;
+	c->bulk_layout = layout;
 	if (runs)
 	{
 		std::vector<double> u(2 * (size_t)S);
@@ -2048,8 +2059,8 @@ int bfvm_set_bulk(bfvm_ctx* c, int64_t count, const int64_t* voxels, const doubl
 		TRY_HIP(c, c->bulk_cell.alloc((size_t)count));
 		TRY_HIP(c, launch_bulk_map(c->bulk_vox.p, count, c->bulk_map.p, c->V, c->stream));
 	}
-	// the last sweep's kernel depends on whether a bulk list exists
-	int rc = had_bulk != (count > 0) ? build_tiles(c) : BFVM_OK;
+	// the last sweep's kernel depends on the bulk list's layout
+	int rc = had_layout != layout ? build_tiles(c) : BFVM_OK;
 	if (!rc)
 		rc = sync_domain(c);
 	return rc ? rc : rebuild_overlap(c);

@@ -1094,12 +1094,13 @@ def test_uniform_bulk_applied_by_last_sweep(dims, grid, S, shape, nearest):
     assert not np.array_equal(fa[j], fb[j])
 
 
-@pytest.mark.parametrize("grid,S,kernels", [((100, 144, 170), 8, [6, 5, 1]), ((170, 100, 190), 8, [6, 1, 5])])
+@pytest.mark.parametrize("grid,S,kernels", [((100, 144, 170), 8, [6, 5, 1]), ((170, 190, 100), 8, [6, 5, 1]),
+                                            ((170, 100, 190), 8, [6, 1, 1])])
 def test_two_lane_mid_lines_past_one_round(grid, S, kernels):
-    """y or z lines of 137..208 voxels whose count needs more than one round
-    of the register file (256 CUs) at one lane per line take two lanes per
-    line; one step matches the oracle (the other axis, within one round,
-    keeps one lane per line)"""
+    """y lines of 137..208 voxels whose count needs more than one round of
+    the register file (256 CUs) at one lane per line take two lanes per line;
+    z lines of that length (the last sweep) keep one lane per line, as do
+    lines within one round; one step matches the oracle"""
     rng = np.random.default_rng(sum(grid) + S)
     p = orc.Problem(3, grid, (20, 20, 20), 0.01, 10 ** rng.uniform(0, 5, S), rng.uniform(0.01, 1, S))
     f0 = rng.uniform(0, 1, p.shape)
@@ -1111,10 +1112,11 @@ def test_two_lane_mid_lines_past_one_round(grid, S, kernels):
 
 
 def test_last_sweep_kernel_follows_the_bulk():
-    """z lines of 209..256 voxels take two lanes per line while the domain has
-    no bulk list, one lane per line (which applies a uniform bulk as it
-    stores) with one: the kernel changes with the list, and both steps match
-    the oracle"""
+    """z lines of 209..256 voxels take two lanes per line unless the bulk list
+    has the shape a uniform bulk riding on the last sweep needs (one run per
+    z line, an eighth of the voxels), whatever its rates; then one lane per
+    line applies it as it stores.  The kernel changes with the list, and both
+    steps match the oracle"""
     from physicore_amd import workloads as wl
 
     w, _, _ = _uniform_bulk_workloads(3, (6, 5, 240), 2, seed=5, shape="dense")
@@ -1135,7 +1137,14 @@ def test_last_sweep_kernel_follows_the_bulk():
     s = orc.Solver(p, field.astype(np.float32).astype(np.float64), dirichlet=dirs, bulk=bulk, agents=ag)
     s.solve(3)
     assert rel_linf(d2.field(), s.field).max() < 1e-5
-    # an empty list: two lanes again
+    # the same voxels with rates that differ at one voxel: still that shape
+    _, wo, _ = _uniform_bulk_workloads(3, (6, 5, 240), 2, seed=5, shape="dense")
+    d.set_bulk(*wo.bulk)
+    assert d.sweep_plan()["kernels"][2] == 1
+    # a few voxels (no such shape), then an empty list: two lanes again
+    vox, sup, upt, tgt = w.bulk
+    d.set_bulk(vox[:20], sup[:20], upt[:20], tgt[:20])
+    assert d.sweep_plan()["kernels"][2] == 5
     d.set_bulk(np.zeros(0, dtype=np.int64), np.zeros((0, 2)), np.zeros((0, 2)), np.zeros((0, 2)))
     assert d.sweep_plan()["kernels"][2] == 5
 
