@@ -529,7 +529,7 @@ int sync_domain(bfvm_ctx* c)
 	d.bulk_a = c->bulk_a.p;
 	d.bulk_r = c->bulk_r.p;
 	c->bulk_fused = c->bulk_runs && c->dims >= 2 && c->fp64 && c->dir_count == 0 &&
-					c->variant[c->dims - 1] == kSweepReg;
+					(c->variant[c->dims - 1] == kSweepReg || (c->variant[c->dims - 1] == kSweepTwist && c->dims == 3));
 	d.bulk_u = c->bulk_fused ? c->bulk_u.p : nullptr;
 	d.bulk_run = c->bulk_fused ? c->bulk_run.p : nullptr;
 	d.bulk_cell = c->bulk_cell.p;
@@ -725,13 +725,14 @@ int build_tiles(bfvm_ctx* c)
 			// 0.096, and at 200^3 x 8, 2.4 rounds, 0.249 / 0.252).  Results
 			// differ by rounding only (4e-7 after 10 steps).  The last sweep
 			// (with the faces) only for lines of 209..256 (200^3 x 4 with an
-			// x-min face: z 46.4 us back to back on one lane, 47.7 on two), and
-			// only when no bulk list has the shape a uniform bulk riding on
-			// the one-lane kernel's stores needs (whatever its rates, so that
-			// the kernels do not change with them)
+			// x-min face: z 46.4 us back to back on one lane, 47.7 on two); z
+			// also applies a uniform bulk as it stores, a 2-D domain's y only
+			// one lane per line does, so there a bulk list of that shape
+			// (judged on its voxels, whatever its rates, so that the kernels
+			// do not change with them) keeps one lane per line
 			else if (axis > 0 && c->fp64 && twist_short_bucket(n) &&
 					 (axis < c->dims - 1 ? n > 208 || (lines > (int64_t)c->num_cu * 512 && lines <= (int64_t)c->num_cu * 1024)
-										 : n > 208 && !c->bulk_layout))
+										 : n > 208 && (axis == 2 || !c->bulk_layout)))
 			{
 				c->variant[axis] = kSweepTwist;
 				c->nmax[axis] = twist_short_bucket(n);

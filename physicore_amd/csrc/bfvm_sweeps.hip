@@ -56,12 +56,12 @@ kernel_fn pick_twist_y_a(int nmax, bool faces);
 kernel_fn pick_twist_y_b(int nmax, bool faces);
 kernel_fn pick_twist_y_c(int nmax, bool faces);
 kernel_fn pick_twist_y_d(int nmax, bool faces);
-kernel_fn pick_twist_z_a(int nmax, bool faces);
-kernel_fn pick_twist_z_b(int nmax, bool faces);
-kernel_fn pick_twist_z_c(int nmax, bool faces);
-kernel_fn pick_twist_z_d(int nmax, bool faces);
+kernel_fn pick_twist_z_a(int nmax, int fin);
+kernel_fn pick_twist_z_b(int nmax, int fin);
+kernel_fn pick_twist_z_c(int nmax, int fin);
+kernel_fn pick_twist_z_d(int nmax, int fin);
 kernel_fn pick_twist_y_e(int nmax, bool faces);
-kernel_fn pick_twist_z_e(int nmax, bool faces);
+kernel_fn pick_twist_z_e(int nmax, int fin);
 kernel_fn pick_xscan(int S, int vpl, bool fp64);
 pipe_fn pick_pipe_x_s1(int nmax, bool faces);
 pipe_fn pick_pipe_x_s2(int nmax, bool faces);
@@ -131,8 +131,9 @@ kernel_fn pick_lds(int axis)
 kernel_fn pick(int axis, int variant, int nmax, bool fp64, int fin, int S)
 {
 	const bool faces = fin != kFinNone;
-	// the bulk-finishing stores exist for the FP64 register y/z kernels only
-	if (fin == kFinBulk && (variant != kSweepReg || !fp64 || axis == 0))
+	// the bulk-finishing stores exist for the FP64 register y/z kernels and
+	// the twisted z kernels only
+	if (fin == kFinBulk && ((variant != kSweepReg && !(variant == kSweepTwist && axis == 2)) || !fp64 || axis == 0))
 		return nullptr;
 	if (variant == kSweepScan)
 		return axis == 0 ? pick_xscan(S, nmax, fp64) : nullptr;
@@ -211,9 +212,12 @@ kernel_fn pick(int axis, int variant, int nmax, bool fp64, int fin, int S)
 		const int g = nmax <= 128 ? 4 : nmax <= 160 ? 0 : nmax <= 192 ? 1 : nmax <= 224 ? 2 : 3;
 		static kernel_fn (*const y[5])(int, bool) = { pick_twist_y_a, pick_twist_y_b, pick_twist_y_c, pick_twist_y_d,
 													   pick_twist_y_e };
-		static kernel_fn (*const z[5])(int, bool) = { pick_twist_z_a, pick_twist_z_b, pick_twist_z_c, pick_twist_z_d,
-													   pick_twist_z_e };
-		return axis == 1 ? y[g](nmax, faces) : z[g](nmax, faces);
+		static kernel_fn (*const z[5])(int, int) = { pick_twist_z_a, pick_twist_z_b, pick_twist_z_c, pick_twist_z_d,
+													  pick_twist_z_e };
+		// a uniform bulk only on the z lines (the last sweep of a 3-D domain)
+		if (fin == kFinBulk && axis == 1)
+			return nullptr;
+		return axis == 1 ? y[g](nmax, faces) : z[g](nmax, fin);
 	}
 	// FP64 instantiations split at 128 lines (two translation units per axis)
 	const bool lo = nmax <= 128;
@@ -372,9 +376,9 @@ hipError_t init_sweep_kernels()
 		}
 	for (int axis = 1; axis < 3; axis++)
 		for (int nmax = twist_short_bucket(kTwistShortMin + 1); nmax <= kRegMaxN; nmax += 8)
-			for (int faces = 0; faces < 2; faces++)
+			for (int fin = 0; fin < (axis == 2 ? 3 : 2); fin++)
 			{
-				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, faces ? kFinFaces : kFinNone, 1);
+				sweep::kernel_fn k = sweep::pick(axis, kSweepTwist, nmax, true, fin, 1);
 				if (!k)
 					return hipErrorInvalidValue;
 				hipError_t e =

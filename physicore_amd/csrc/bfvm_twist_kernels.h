@@ -50,6 +50,12 @@ struct TwistLine
 	int n, m, nb;	   // line length, top rows, bottom rows
 	double cc;		 // c of the substrate
 	LineKey kf, ko;	 // rounding keys (forward values, results)
+	// the last sweep's uniform bulk (BULK): (v + ba) * br over rows r0 ..
+	// r0 + rlen - 1, rounded with the bulk stage's key of the line (kb) and row
+	double ba, br;
+	int r0;
+	unsigned rlen;
+	LineKey kb;
 	// LDS element offsets (integers, so that the LDS address space stays visible)
 	int tb;		// tb[k] = b'_k, tb[-16..-1] = 0
 	int ifc;	// this half's interface coefficients (own d', other d')
@@ -57,9 +63,18 @@ struct TwistLine
 	int theirs; // the partner lane's
 };
 
+// The bulk stage on a final value of row r (bulk_round's bits): formed at
+// every row and picked by a mask, as in the one-lane kernels.
+__device__ __forceinline__ float twist_bulk(const TwistLine& L, int r, float x)
+{
+	const float v = field_round<double>(((double)x + L.ba) * L.br, L.kb, weyl29((uint32_t)r));
+	const unsigned m = 0u - (unsigned)((unsigned)(r - L.r0) < L.rlen);
+	return __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, v) & m) | (__builtin_bit_cast(unsigned, x) & ~m));
+}
+
 // One half of a line: load, eliminate towards the interface, exchange, solve
 // the 2x2 interface, substitute back.  Every wave reaches the one barrier.
-template <bool BOTTOM, int NH>
+template <bool BOTTOM, int NH, bool BULK = false>
 __device__ __forceinline__ void twist_half(const TwistLine& L, double* lds)
 {
 	constexpr int nchunks = (NH - 1 + kTwistChunk - 1) / kTwistChunk;
@@ -163,7 +178,9 @@ __device__ __forceinline__ void twist_half(const TwistLine& L, double* lds)
 		const double* lb = lds + tbo;
 		auto out = [&](auto J, double x) {
 			constexpr int j = J;
-			const float val = field_round<double>(x, L.ko, weyl29(j));
+			float val = field_round<double>(x, L.ko, weyl29(j));
+			if constexpr (BULK)
+				val = twist_bulk(L, j - pad, val);
 			if constexpr (j < kTwistPrefix) // pad <= 16: later slots are real rows
 			{
 				if (j >= pad)
@@ -193,7 +210,9 @@ __device__ __forceinline__ void twist_half(const TwistLine& L, double* lds)
 		const double* db = lds + tbo;
 		auto out = [&](auto J, double x) {
 			constexpr int j = J;
-			const float val = field_round<double>(x, L.ko, weyl29(j));
+			float val = field_round<double>(x, L.ko, weyl29(j));
+			if constexpr (BULK)
+				val = twist_bulk(L, sm + j, val);
 			if constexpr (j >= NH - kTwistPrefix - 1) // nb >= NH - 16: earlier slots are real rows
 			{
 				if (j < snb)
@@ -217,7 +236,10 @@ __device__ __forceinline__ void twist_half(const TwistLine& L, double* lds)
 	}
 }
 
-template <int AXIS, int NH, bool FACES>
+// BULK (with FACES): the launch also applies the last sweep's uniform bulk
+// as it stores (DomainDev::bulk_run), after the faces, with the arithmetic and
+// bits of the bulk stage on the stored field.
+template <int AXIS, int NH, bool FACES, bool BULK = false>
 __global__ __launch_bounds__(kSweepThreads) void sweep_twist_kernel(const DomainDev* __restrict__ doms,
 																	const TileRef* __restrict__ tiles, int ntiles,
 																	int finish, int fac_cap)
@@ -293,10 +315,27 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_twist_kernel(const Domain
 		T.ifc = 2 * fac_cap + 64 + 4 * cs + (bottom ? 2 : 0);
 		T.mine = xbase + (int)threadIdx.x;
 		T.theirs = xbase + ((int)threadIdx.x ^ 64);
+		if constexpr (BULK)
+		{
+			const double* bu = finish && AXIS == d.dims - 1 ? d.bulk_u : nullptr;
+			const int sub = c % S;
+			const int col = f / S;
+			T.ba = bu ? bu[sub] : 0.0;
+			T.br = bu ? bu[S + sub] : 1.0;
+			T.r0 = bu ? d.bulk_run[2 * col] : 0;
+			T.rlen = bu ? (unsigned)(d.bulk_run[2 * col + 1] - T.r0) : 0u;
+			T.kb = line_key(d, kRoundBulk, (uint32_t)f);
+		}
 		if (bottom)
-			twist_half<true, NH>(T, lds);
+			twist_half<true, NH, BULK>(T, lds);
 		else
-			twist_half<false, NH>(T, lds);
+			twist_half<false, NH, BULK>(T, lds);
+		auto fin = [&](int r, float x) -> float {
+			if constexpr (BULK)
+				return twist_bulk(T, r, x);
+			else
+				return x;
+		};
 		// Dirichlet faces after the solve (face values replace solved ones, and
 		// this lane's later stores win), so the solve carries no face state:
 		// row 0 / row n-1 of this half, or every row of a line lying in a face
@@ -311,12 +350,12 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_twist_kernel(const Domain
 				if (lf.lo || lf.hi)
 				{
 					for (int r = r0; r < r1; r++)
-						buf_store(T.rsrc, T.voff, r * T.stride4, face_value(lf, r, n, 0.0f));
+						buf_store(T.rsrc, T.voff, r * T.stride4, fin(r, face_value(lf, r, n, 0.0f)));
 				}
 				else if (!bottom && lf.has_min)
-					buf_store(T.rsrc, T.voff, 0, lf.min_v);
+					buf_store(T.rsrc, T.voff, 0, fin(0, lf.min_v));
 				else if (bottom && lf.has_max)
-					buf_store(T.rsrc, T.voff, (n - 1) * T.stride4, lf.max_v);
+					buf_store(T.rsrc, T.voff, (n - 1) * T.stride4, fin(n - 1, lf.max_v));
 			}
 	}
 }

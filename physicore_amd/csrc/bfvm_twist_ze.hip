@@ -5,11 +5,13 @@
 namespace bfvm {
 namespace sweep {
 
-kernel_fn pick_twist_z_e(int nmax, bool faces)
+kernel_fn pick_twist_z_e(int nmax, int fin)
 {
 #define BFVM_BUCKET(N) \
 	case N:            \
-		return faces ? sweep_twist_kernel<2, N, true> : sweep_twist_kernel<2, N, false>;
+		return fin == kFinBulk	? sweep_twist_kernel<2, N, true, true>  \
+			   : fin == kFinFaces ? sweep_twist_kernel<2, N, true, false> \
+								  : sweep_twist_kernel<2, N, false, false>;
 	switch (nmax)
 	{
 		BFVM_BUCKET(72)
